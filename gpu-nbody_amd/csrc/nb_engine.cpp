@@ -1,0 +1,537 @@
+// nb_engine.cpp -- host side of the engine and the C ABI of include/nbody.h.
+//
+// Replaces GPUBarnesHutNBodySimulation (the reference's host driver): device
+// selection, node-pool sizing, buffer allocation, the per-step launch order
+// and the error checks. One HIP stream per handle at default priority; a step
+// is six launches with no host synchronization between them, and nb_step
+// reads the device error word once at the end.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbody.h"
+#include "nb_internal.h"
+
+struct nb_sim {
+    nb_params p;
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    nb::Dev d{};
+    int bbox_blocks = 1, build_blocks = 1, summarize_blocks = 1, sort_blocks = 1;
+    bool have_state = false;
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    double stage_ms[8] = {};
+    unsigned long long inter_before = 0;
+    long long last_interactions = 0;
+    long long fallbacks = 0;
+    double last_check_median = 0.0;
+    std::string err;
+};
+
+namespace {
+
+// message of the last nb_create that failed on this thread (nb_last_error(NULL))
+thread_local std::string g_create_error;
+
+int fail(nb_sim *s, int code, const char *fmt, ...)
+{
+    if (s) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        s->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(sim, expr)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(sim, NB_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));            \
+    } while (0)
+
+// Build threads: each inserts >= 16 bodies; capped so that the cells lost to
+// publication races (at most a few per thread) stay a small slack on the pool.
+int build_threads(int n) { return std::min(131072, std::max(256, (n / 16 + 255) / 256 * 256)); }
+
+// Cell pool: the reference's NUMBER_OF_NODES = 2N (java:219-227) scaled by
+// pool_factor / 2, plus 8 cells per build thread for unpublished spares.
+long long pool_cells(const nb_params *p)
+{
+    const double base = std::max(16384.0, std::ceil((double)p->pool_factor * p->n_bodies));
+    return (long long)base + 8LL * build_threads(p->n_bodies);
+}
+
+int validate(const nb_params *p)
+{
+    if (!p) return NB_E_ARG;
+    if (p->n_bodies < 1) return NB_E_ARG;
+    if (!(p->theta >= 0.0f) || !std::isfinite(p->theta)) return NB_E_ARG;
+    if (!(p->softening > 0.0f) || !std::isfinite(p->softening)) return NB_E_ARG;
+    if (!std::isfinite(p->dt)) return NB_E_ARG;
+    if (p->vote_width != 16 && p->vote_width != 32 && p->vote_width != 64) return NB_E_ARG;
+    if (p->flags & ~(NB_FP64_POSITIONS | NB_FUSED_UPDATE)) return NB_E_ARG;
+    if (!(p->pool_factor >= 1.0f) || p->pool_factor > 64.0f) return NB_E_ARG;
+    if (p->mode != NB_MODE_PLAIN && p->mode != NB_MODE_LEARNED) return NB_E_ARG;
+    if (p->mode == NB_MODE_LEARNED && !(p->coarse_theta >= p->theta)) return NB_E_ARG;
+    // the traversal stack packs (cell << 6 | depth) into 32 bits
+    if ((double)pool_cells(p) > (double)(1 << 26)) return NB_E_ARG;
+    return NB_OK;
+}
+
+template <class T> hipError_t dalloc(T **p, size_t count)
+{
+    return hipMalloc((void **)p, count * sizeof(T) + 64);
+}
+
+nb::Ctl initial_ctl(const nb_sim *s, uint32_t epoch)
+{
+    nb::Ctl c{};
+    c.step = -1;     // GPUBarnesHutNBodySimulation.java:167
+    c.epoch = epoch;
+    c.n_cells = 1;
+    c.max_depth = 1; // GPUBarnesHutNBodySimulation.java:170
+    c.error = 0;
+    c.bbox_count = 0;
+    c.interactions = 0;
+    c.spin_limit = 1 << 20;
+    (void)s;
+    return c;
+}
+
+int read_ctl(nb_sim *s, nb::Ctl *out)
+{
+    HIP_TRY(s, hipMemcpyAsync(out, s->d.ctl, sizeof(nb::Ctl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int check_device_error(nb_sim *s)
+{
+    nb::Ctl c;
+    int r = read_ctl(s, &c);
+    if (r) return r;
+    s->last_interactions = (long long)(c.interactions - s->inter_before);
+    s->inter_before = c.interactions;
+    switch (c.error) {
+    case nb::kErrNone: return NB_OK;
+    case nb::kErrPool: return fail(s, NB_E_POOL, "octree cell pool exhausted (%d cells)", s->d.cap);
+    case nb::kErrDepth: return fail(s, NB_E_DEPTH, "octree deeper than %d levels", NB_MAX_DEPTH);
+    case nb::kErrSync: return fail(s, NB_E_SYNC, "bounded device wait ran out (inconsistent tree)");
+    case nb::kErrStack: return fail(s, NB_E_STACK, "force traversal stack overflow");
+    default: return fail(s, NB_E_HIP, "unknown device error %d", c.error);
+    }
+}
+
+int launch_stage(nb_sim *s, int stage)
+{
+    const nb::Dev &d = s->d;
+    hipError_t e = hipSuccess;
+    const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
+    switch (stage) {
+    case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
+    case NB_STAGE_BUILD: e = nb::launch_build(d, s->build_blocks, s->stream); break;
+    case NB_STAGE_SUMMARIZE: e = nb::launch_summarize(d, s->summarize_blocks, s->stream); break;
+    case NB_STAGE_SORT: e = nb::launch_sort(d, s->sort_blocks, s->stream); break;
+    case NB_STAGE_FORCE: e = nb::launch_force(d, s->p.vote_width, fused, s->stream); break;
+    case NB_STAGE_INTEGRATE:
+        if (!fused) e = nb::launch_integrate(d, s->stream);
+        break;
+    default: return fail(s, NB_E_ARG, "stage %d out of range 1..6", stage);
+    }
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "stage %d launch: %s", stage, hipGetErrorString(e));
+    return NB_OK;
+}
+
+void free_all(nb_sim *s)
+{
+    nb::Dev &d = s->d;
+    void *ptrs[] = {d.ctl,   d.x,     d.y,      d.z,     d.vx,    d.vy,    d.vz,     d.ax,
+                    d.ay,    d.az,    d.m,      d.x64,   d.y64,   d.z64,   d.slot,   d.ready,
+                    d.sready, d.count, d.start, d.com64, d.cnode, d.cchild, d.sorted, d.bnode,
+                    d.bbox_part};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+} // namespace
+
+extern "C" {
+
+void nb_default_params(nb_params *p, int32_t n_bodies)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->n_bodies = n_bodies;
+    p->theta = std::sqrt(1.5f); // THETA 1.5 = theta^2 (calculateforce.cl:16)
+    p->softening = 0.05f;       // EPSILON 0.0025 = softening^2 (calculateforce.cl:14)
+    p->dt = 0.025f;             // TIMESTEP (calculateforce.cl:18)
+    p->vote_width = 16;         // WARPSIZE (GPUBarnesHutNBodySimulation.java:47)
+    p->flags = 0;
+    p->device = 0;
+    p->pool_factor = 2.0f;      // NUMBER_OF_NODES = 2 * NBODIES (java:220)
+    p->mode = NB_MODE_PLAIN;
+    p->coarse_theta = 0.9f;
+    p->check_tol = 1e-3f;
+    p->check_every = 0;
+    p->check_samples = 4096;
+}
+
+int nb_create(const nb_params *p, nb_sim **out)
+{
+    if (!out) return NB_E_ARG;
+    *out = nullptr;
+    int v = validate(p);
+    if (v) {
+        g_create_error = "invalid nb_params";
+        return v;
+    }
+    nb_sim *s = new nb_sim();
+    s->p = *p;
+    const int n = p->n_bodies;
+    auto bail = [&](int code) {
+        g_create_error = s->err.empty() ? nb_status_string(code) : s->err;
+        free_all(s);
+        delete s;
+        return code;
+    };
+    hipError_t e = hipSetDevice(p->device);
+    if (e != hipSuccess) {
+        fail(s, NB_E_HIP, "hipSetDevice(%d): %s", p->device, hipGetErrorString(e));
+        return bail(NB_E_HIP);
+    }
+    s->device = p->device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) return bail(NB_E_HIP);
+    s->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail(NB_E_HIP);
+
+    nb::Dev &d = s->d;
+    d.n = n;
+    d.cap = (int)pool_cells(p);
+    d.theta2 = p->theta * p->theta;       // THETA = theta^2
+    d.eps2 = p->softening * p->softening; // EPSILON = softening^2
+    d.dt = p->dt;
+    d.fp64 = (p->flags & NB_FP64_POSITIONS) ? 1 : 0;
+    const size_t cap = (size_t)d.cap;
+    bool ok = dalloc(&d.ctl, 1) == hipSuccess;
+    for (float **f : {&d.x, &d.y, &d.z, &d.vx, &d.vy, &d.vz, &d.ax, &d.ay, &d.az, &d.m})
+        ok = ok && dalloc(f, (size_t)n) == hipSuccess;
+    if (d.fp64)
+        for (double **f : {&d.x64, &d.y64, &d.z64}) ok = ok && dalloc(f, (size_t)n) == hipSuccess;
+    ok = ok && dalloc(&d.slot, 8 * cap) == hipSuccess;
+    ok = ok && dalloc(&d.ready, cap) == hipSuccess && dalloc(&d.sready, cap) == hipSuccess;
+    ok = ok && dalloc(&d.count, cap) == hipSuccess && dalloc(&d.start, cap) == hipSuccess;
+    ok = ok && dalloc(&d.com64, cap) == hipSuccess && dalloc(&d.cnode, cap) == hipSuccess;
+    ok = ok && dalloc(&d.cchild, 8 * cap) == hipSuccess;
+    ok = ok && dalloc(&d.sorted, (size_t)n) == hipSuccess && dalloc(&d.bnode, (size_t)n) == hipSuccess;
+    ok = ok && dalloc(&d.bbox_part, (size_t)6 * nb::kBBoxBlocks) == hipSuccess;
+    if (!ok) {
+        fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
+        return bail(NB_E_HIP);
+    }
+    // epoch 0 tags every slot and flag as stale; the first bounding box moves to epoch 1
+    ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.ready, 0, cap * sizeof(uint32_t), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.sready, 0, cap * sizeof(uint32_t), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.count, 0, cap * sizeof(int), s->stream) == hipSuccess;
+    nb::Ctl c = initial_ctl(s, 0);
+    ok = ok && hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(s->stream) == hipSuccess;
+    if (!ok) return bail(NB_E_HIP);
+    for (auto &ev : s->ev)
+        if (hipEventCreate(&ev) != hipSuccess) return bail(NB_E_HIP);
+
+    s->bbox_blocks = std::max(1, std::min(nb::kBBoxBlocks, (n + 2047) / 2048));
+    s->build_blocks = build_threads(n) / 256;
+    // summarize and sort poll on each other's results inside one launch, so
+    // every block of the grid must be resident: stay at half the occupancy
+    // limit and at most 4 blocks per CU
+    const int sb = std::max(1, std::min(4, nb::summarize_blocks_per_cu(s->device) / 2));
+    const int tb = std::max(1, std::min(4, nb::sort_blocks_per_cu(s->device) / 2));
+    s->summarize_blocks = sb * s->cus;
+    s->sort_blocks = tb * s->cus;
+    *out = s;
+    return NB_OK;
+}
+
+void nb_destroy(nb_sim *s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+}
+
+int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, const float *vx,
+                 const float *vy, const float *vz, const float *m)
+{
+    if (!s) return NB_E_ARG;
+    if (!x || !y || !z || !vx || !vy || !vz || !m) return fail(s, NB_E_ARG, "null state array");
+    HIP_TRY(s, hipSetDevice(s->device));
+    nb::Dev &d = s->d;
+    const size_t bytes = (size_t)d.n * sizeof(float);
+    const float *src[] = {x, y, z, vx, vy, vz, m};
+    float *dst[] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m};
+    for (int i = 0; i < 7; ++i)
+        HIP_TRY(s, hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyHostToDevice, s->stream));
+    for (float *a : {d.ax, d.ay, d.az}) HIP_TRY(s, hipMemsetAsync(a, 0, bytes, s->stream));
+    if (d.fp64) {
+        std::vector<double> tmp((size_t)d.n);
+        const float *src3[] = {x, y, z};
+        double *dst3[] = {d.x64, d.y64, d.z64};
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < d.n; ++i) tmp[i] = (double)src3[k][i];
+            HIP_TRY(s, hipMemcpy(dst3[k], tmp.data(), (size_t)d.n * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    // keep the epoch running so that no slot of an earlier tree can look current
+    nb::Ctl cur;
+    int r = read_ctl(s, &cur);
+    if (r) return r;
+    nb::Ctl c = initial_ctl(s, cur.epoch);
+    HIP_TRY(s, hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->inter_before = 0;
+    s->have_state = true;
+    s->err.clear();
+    return NB_OK;
+}
+
+int nb_run_stage(nb_sim *s, int32_t stage)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    int r = launch_stage(s, stage);
+    if (r) return r;
+    return check_device_error(s);
+}
+
+int nb_step(nb_sim *s, int32_t nsteps)
+{
+    if (!s) return NB_E_ARG;
+    if (nsteps < 0) return fail(s, NB_E_ARG, "negative step count");
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    for (int k = 0; k < nsteps; ++k) {
+        const bool t = s->timing && k == nsteps - 1;
+        for (int st = NB_STAGE_BBOX; st <= NB_STAGE_INTEGRATE; ++st) {
+            if (t) HIP_TRY(s, hipEventRecord(s->ev[st - 1], s->stream));
+            int r = launch_stage(s, st);
+            if (r) return r;
+        }
+        if (t) HIP_TRY(s, hipEventRecord(s->ev[6], s->stream));
+    }
+    int r = check_device_error(s);
+    if (s->timing && nsteps > 0) {
+        for (int st = 0; st < 6; ++st) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, s->ev[st], s->ev[st + 1]) == hipSuccess) s->stage_ms[st] = ms;
+        }
+    }
+    return r;
+}
+
+int nb_copy_vertices(nb_sim *s, float *pos4, float *vel4)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t bytes = (size_t)s->d.n * sizeof(float4);
+    float4 *dp = nullptr, *dv = nullptr;
+    if (pos4) HIP_TRY(s, hipMalloc((void **)&dp, bytes));
+    if (vel4) {
+        hipError_t e = hipMalloc((void **)&dv, bytes);
+        if (e != hipSuccess) {
+            if (dp) (void)hipFree(dp);
+            return fail(s, NB_E_HIP, "hipMalloc: %s", hipGetErrorString(e));
+        }
+    }
+    hipError_t e = nb::launch_copy_vertices(s->d, dp, dv, s->stream);
+    if (e == hipSuccess && dp) e = hipMemcpyAsync(pos4, dp, bytes, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess && dv) e = hipMemcpyAsync(vel4, dv, bytes, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (dp) (void)hipFree(dp);
+    if (dv) (void)hipFree(dv);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "copy_vertices: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+int32_t nb_num_bodies(const nb_sim *s) { return s ? s->d.n : 0; }
+
+static int copy_back(nb_sim *s, void *dst, const void *src, size_t bytes)
+{
+    if (!dst) return NB_OK;
+    HIP_TRY(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+    return NB_OK;
+}
+
+int nb_get_state(nb_sim *s, float *x, float *y, float *z, float *vx, float *vy, float *vz)
+{
+    if (!s) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t b = (size_t)s->d.n * sizeof(float);
+    int r = 0;
+    r |= copy_back(s, x, s->d.x, b);
+    r |= copy_back(s, y, s->d.y, b);
+    r |= copy_back(s, z, s->d.z, b);
+    r |= copy_back(s, vx, s->d.vx, b);
+    r |= copy_back(s, vy, s->d.vy, b);
+    r |= copy_back(s, vz, s->d.vz, b);
+    if (r) return NB_E_HIP;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int nb_get_accel(nb_sim *s, float *ax, float *ay, float *az)
+{
+    if (!s) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t b = (size_t)s->d.n * sizeof(float);
+    int r = copy_back(s, ax, s->d.ax, b) | copy_back(s, ay, s->d.ay, b) | copy_back(s, az, s->d.az, b);
+    if (r) return NB_E_HIP;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int nb_get_positions64(nb_sim *s, double *x, double *y, double *z)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->d.fp64) return fail(s, NB_E_STATE, "NB_FP64_POSITIONS is not set");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t b = (size_t)s->d.n * sizeof(double);
+    int r = copy_back(s, x, s->d.x64, b) | copy_back(s, y, s->d.y64, b) | copy_back(s, z, s->d.z64, b);
+    if (r) return NB_E_HIP;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int nb_get_root(nb_sim *s, float *out4)
+{
+    if (!s || !out4) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    nb::Ctl c;
+    int r = read_ctl(s, &c);
+    if (r) return r;
+    out4[0] = c.root[0];
+    out4[1] = c.root[1];
+    out4[2] = c.root[2];
+    out4[3] = c.radius;
+    return NB_OK;
+}
+
+int nb_get_sorted(nb_sim *s, int32_t *sorted)
+{
+    if (!s || !sorted) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipMemcpyAsync(sorted, s->d.sorted, (size_t)s->d.n * sizeof(int), hipMemcpyDeviceToHost,
+                              s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int nb_get_paths(nb_sim *s, uint64_t *paths3, int32_t *depth)
+{
+    if (!s || !paths3 || !depth) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t n = (size_t)s->d.n;
+    unsigned long long *dp = nullptr;
+    int *dd = nullptr;
+    HIP_TRY(s, hipMalloc((void **)&dp, 3 * n * sizeof(unsigned long long)));
+    hipError_t e = hipMalloc((void **)&dd, n * sizeof(int));
+    if (e == hipSuccess) e = nb::launch_paths(s->d, dp, dd, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(paths3, dp, 3 * n * 8, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(depth, dd, n * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dp);
+    if (dd) (void)hipFree(dd);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "get_paths: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+int nb_get_cells(nb_sim *s, int32_t cap, int32_t *ncells, int32_t *count, float *x, float *y, float *z,
+                 float *m)
+{
+    if (!s || !ncells) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    nb::Ctl c;
+    int r = read_ctl(s, &c);
+    if (r) return r;
+    const int C = std::min(c.n_cells, s->d.cap);
+    *ncells = C;
+    if (cap < C) return fail(s, NB_E_ARG, "cell buffers hold %d of %d cells", cap, C);
+    std::vector<float4> node((size_t)C);
+    if (count) HIP_TRY(s, hipMemcpyAsync(count, s->d.count, (size_t)C * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(node.data(), s->d.cnode, (size_t)C * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    for (int i = 0; i < C; ++i) {
+        if (x) x[i] = node[i].x;
+        if (y) y[i] = node[i].y;
+        if (z) z[i] = node[i].z;
+        if (m) m[i] = node[i].w;
+    }
+    return NB_OK;
+}
+
+int nb_get_counters(nb_sim *s, nb_counters *out)
+{
+    if (!s || !out) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    nb::Ctl c;
+    int r = read_ctl(s, &c);
+    if (r) return r;
+    std::memset(out, 0, sizeof *out);
+    out->step = c.step;
+    out->cells = c.n_cells;
+    out->max_depth = c.max_depth;
+    out->live_cells = c.live_cells;
+    out->interactions = s->last_interactions;
+    out->error = c.error;
+    out->fallbacks = s->fallbacks;
+    for (int i = 0; i < 8; ++i) out->stage_ms[i] = s->stage_ms[i];
+    out->last_check_median = s->last_check_median;
+    return NB_OK;
+}
+
+int nb_set_timing(nb_sim *s, int32_t enabled)
+{
+    if (!s) return NB_E_ARG;
+    s->timing = enabled != 0;
+    return NB_OK;
+}
+
+const char *nb_last_error(const nb_sim *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+const char *nb_status_string(int code)
+{
+    switch (code) {
+    case NB_OK: return "ok";
+    case NB_E_HIP: return "HIP runtime error";
+    case NB_E_POOL: return "octree cell pool exhausted";
+    case NB_E_DEPTH: return "octree too deep";
+    case NB_E_ARG: return "invalid argument";
+    case NB_E_SYNC: return "bounded device wait ran out";
+    case NB_E_STATE: return "call out of order";
+    case NB_E_STACK: return "traversal stack overflow";
+    case NB_E_MODEL: return "learned model missing or malformed";
+    default: return "unknown status";
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,241 @@
+"""Host-side mirror of the reference's simulation plugin API.
+
+AbstractNBodySimulation, its Mode enum and the UniverseGenerator interface keep
+the reference's names and call order (AbstractNBodySimulation.java:5-44,
+UniverseGenerator.java:7-23); HipBarnesHutNBodySimulation takes the place of
+GPUBarnesHutNBodySimulation (GPUBarnesHutNBodySimulation.java:38-448) and
+drives libnbody.so through its C ABI. As in the reference, where OpenCL API
+errors throw (java:40-42), every failing call raises NbError.
+"""
+from __future__ import annotations
+
+import abc
+import ctypes as C
+import enum
+from typing import Optional, Protocol
+
+import numpy as np
+
+from . import _lib
+from ._lib import NbCounters, NbError, NbParams, ptr
+
+
+class Mode(enum.Enum):
+    """AbstractNBodySimulation.Mode (AbstractNBodySimulation.java:41-43)."""
+
+    GL_INTEROP = 0
+    DEFAULT = 1
+
+
+class UniverseGenerator(Protocol):
+    """UniverseGenerator.generate (UniverseGenerator.java:22): fill bodies
+    [offset, offset + nbodies) of the given arrays."""
+
+    def generate(self, offset: int, nbodies: int, bodiesX: np.ndarray, bodiesY: np.ndarray,
+                 bodiesZ: np.ndarray, velX: np.ndarray, velY: np.ndarray, velZ: np.ndarray,
+                 bodiesMass: np.ndarray) -> None: ...
+
+
+class AbstractNBodySimulation(abc.ABC):
+    """AbstractNBodySimulation (AbstractNBodySimulation.java:5-44)."""
+
+    def __init__(self, mode: Mode):
+        self.mode = mode
+
+    @abc.abstractmethod
+    def step(self) -> None:
+        """Executes the next step of the simulation."""
+
+    @abc.abstractmethod
+    def initGLBuffers(self, gl, positionVBO: int, velocityVBO: int) -> None:
+        """Initializes the position and velocity buffers for the bodies."""
+
+    @abc.abstractmethod
+    def getNumberOfBodies(self) -> int:
+        """Returns the number of bodies for this simulation."""
+
+    @abc.abstractmethod
+    def init(self, gl) -> None:
+        """Initializes the simulation."""
+
+
+def _check(lib, handle, code):
+    if code != _lib.NB_OK:
+        msg = lib.nb_last_error(handle)
+        raise NbError(code, (msg.decode() if msg else "") or lib.nb_status_string(code).decode())
+    return code
+
+
+def make_params(nbodies: int, *, theta: Optional[float] = None, softening: Optional[float] = None,
+                dt: Optional[float] = None, vote_width: Optional[int] = None, flags: int = 0,
+                device: int = 0, pool_factor: Optional[float] = None, mode: int = _lib.NB_MODE_PLAIN,
+                coarse_theta: Optional[float] = None, check_tol: Optional[float] = None,
+                check_every: Optional[int] = None, check_samples: Optional[int] = None) -> NbParams:
+    """nb_params with the reference's defaults (nb_default_params) and overrides."""
+    lib = _lib.load()
+    p = NbParams()
+    lib.nb_default_params(C.byref(p), int(nbodies))
+    for name, val in (("theta", theta), ("softening", softening), ("dt", dt),
+                      ("vote_width", vote_width), ("pool_factor", pool_factor),
+                      ("coarse_theta", coarse_theta), ("check_tol", check_tol),
+                      ("check_every", check_every), ("check_samples", check_samples)):
+        if val is not None:
+            setattr(p, name, val)
+    p.flags = int(flags)
+    p.device = int(device)
+    p.mode = int(mode)
+    return p
+
+
+class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
+    """Barnes-Hut simulation on one MI355X through libnbody.so.
+
+    Constructor arguments follow GPUBarnesHutNBodySimulation(mode, nbodies,
+    generator) (java:104-108); the physics constants that the reference
+    compiles into its kernels (calculateforce.cl:11-18) are keyword arguments
+    here, defaulting to the same values.
+    """
+
+    def __init__(self, mode: Mode, nbodies: int, generator: UniverseGenerator, **params):
+        super().__init__(mode)
+        self.nbodies = int(nbodies)
+        self.universeGenerator = generator
+        self._params = make_params(self.nbodies, **params)
+        self._lib = _lib.load()
+        self._h = C.c_void_p()
+        self._gl = None
+        self.positionVBO = -1
+        self.velocityVBO = -1
+        self.positions: Optional[np.ndarray] = None
+        self.velocities: Optional[np.ndarray] = None
+
+    # -- AbstractNBodySimulation ------------------------------------------------
+    def init(self, gl=None) -> None:
+        """GPUBarnesHutNBodySimulation.init (java:111-151): create the device
+        state and upload the generated universe."""
+        lib = self._lib
+        code = lib.nb_create(C.byref(self._params), C.byref(self._h))
+        if code != _lib.NB_OK:
+            msg = lib.nb_last_error(None)
+            raise NbError(code, (msg.decode() if msg else "") or lib.nb_status_string(code).decode())
+        n = self.nbodies
+        arrs = [np.zeros(n, np.float32) for _ in range(7)]
+        self.universeGenerator.generate(0, n, *arrs)
+        self.set_state(*arrs)
+
+    def initGLBuffers(self, gl, positionVBO: int, velocityVBO: int) -> None:
+        """GPUBarnesHutNBodySimulation.initGLBuffers (java:229-246). There is no
+        CL-GL sharing on the GPU side: the simulation keeps host float4 copies of
+        the vertices and, in GL_INTEROP mode, uploads them into the given VBOs
+        after every step (gl.glBufferSubData)."""
+        self._gl = gl
+        self.positionVBO = positionVBO
+        self.velocityVBO = velocityVBO
+        n = self.nbodies
+        self.positions = np.zeros((n, 4), np.float32)
+        self.velocities = np.zeros((n, 4), np.float32)
+
+    def step(self) -> None:
+        """GPUBarnesHutNBodySimulation.step (java:249-271)."""
+        _check(self._lib, self._h, self._lib.nb_step(self._h, 1))
+        if self.mode == Mode.GL_INTEROP and self.positions is not None:
+            self.copy_vertices(self.positions, self.velocities)
+            if self._gl is not None:
+                self._upload(self._gl)
+
+    def getNumberOfBodies(self) -> int:
+        return self.nbodies
+
+    # -- engine extras ----------------------------------------------------------
+    def _upload(self, gl) -> None:
+        GL_ARRAY_BUFFER = 0x8892
+        gl.glBindBuffer(GL_ARRAY_BUFFER, self.positionVBO)
+        gl.glBufferSubData(GL_ARRAY_BUFFER, 0, self.positions.nbytes, self.positions)
+        gl.glBindBuffer(GL_ARRAY_BUFFER, self.velocityVBO)
+        gl.glBufferSubData(GL_ARRAY_BUFFER, 0, self.velocities.nbytes, self.velocities)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.nb_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, x, y, z, vx, vy, vz, m) -> None:
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        _check(self._lib, self._h, self._lib.nb_set_state(self._h, f(x), f(y), f(z), f(vx), f(vy), f(vz), f(m)))
+
+    def steps(self, k: int) -> None:
+        _check(self._lib, self._h, self._lib.nb_step(self._h, int(k)))
+
+    def run_stage(self, stage: int) -> None:
+        _check(self._lib, self._h, self._lib.nb_run_stage(self._h, int(stage)))
+
+    def copy_vertices(self, pos4: Optional[np.ndarray], vel4: Optional[np.ndarray]) -> None:
+        _check(self._lib, self._h, self._lib.nb_copy_vertices(self._h, ptr(pos4), ptr(vel4)))
+
+    def state(self):
+        n = self.nbodies
+        a = [np.zeros(n, np.float32) for _ in range(6)]
+        _check(self._lib, self._h, self._lib.nb_get_state(self._h, *[ptr(x) for x in a]))
+        return a
+
+    def accel(self) -> np.ndarray:
+        n = self.nbodies
+        a = [np.zeros(n, np.float32) for _ in range(3)]
+        _check(self._lib, self._h, self._lib.nb_get_accel(self._h, *[ptr(x) for x in a]))
+        return np.stack(a, axis=1)
+
+    def positions64(self) -> np.ndarray:
+        n = self.nbodies
+        a = [np.zeros(n, np.float64) for _ in range(3)]
+        _check(self._lib, self._h, self._lib.nb_get_positions64(self._h, *[ptr(x) for x in a]))
+        return np.stack(a, axis=1)
+
+    def root(self) -> np.ndarray:
+        out = np.zeros(4, np.float32)
+        _check(self._lib, self._h, self._lib.nb_get_root(self._h, out))
+        return out
+
+    def sorted(self) -> np.ndarray:
+        out = np.zeros(self.nbodies, np.int32)
+        _check(self._lib, self._h, self._lib.nb_get_sorted(self._h, out))
+        return out
+
+    def paths(self):
+        p = np.zeros((self.nbodies, 3), np.uint64)
+        d = np.zeros(self.nbodies, np.int32)
+        _check(self._lib, self._h, self._lib.nb_get_paths(self._h, p.reshape(-1), d))
+        return p, d
+
+    def cells(self):
+        nc = C.c_int32(0)
+        cap = int(max(self.counters().cells, 1))
+        cnt = np.zeros(cap, np.int32)
+        xs = [np.zeros(cap, np.float32) for _ in range(4)]
+        _check(self._lib, self._h, self._lib.nb_get_cells(self._h, cap, C.byref(nc), ptr(cnt),
+                                                          *[ptr(a) for a in xs]))
+        k = nc.value
+        return dict(count=cnt[:k], x=xs[0][:k], y=xs[1][:k], z=xs[2][:k], m=xs[3][:k])
+
+    def counters(self) -> NbCounters:
+        c = NbCounters()
+        _check(self._lib, self._h, self._lib.nb_get_counters(self._h, C.byref(c)))
+        return c
+
+    def set_timing(self, on: bool) -> None:
+        _check(self._lib, self._h, self._lib.nb_set_timing(self._h, 1 if on else 0))

@@ -1,0 +1,176 @@
+/*
+ * nbody.h -- C ABI of the MI355X Barnes-Hut engine (libnbody.so).
+ *
+ * A drop-in for the kernels/nbody pipeline of bneukom/gpu-nbody. The entry
+ * points replace what the reference's host class exposes through its plugin
+ * API (src/ch/fhnw/woipv/nbody/simulation/AbstractNBodySimulation.java:5-44)
+ * and what GPUBarnesHutNBodySimulation does behind it; each declaration names
+ * the reference interface it replaces. Plain C types only: an opaque handle,
+ * pointers and sizes. Host arrays passed in are copied before the call
+ * returns; no pointer is retained.
+ *
+ * Threading: one handle is not thread-safe; calls on a handle are serialized
+ * by the caller. Distinct handles are independent. Every call that runs GPU
+ * work is synchronous on the handle's own HIP stream.
+ *
+ * Errors: every int-returning call returns NB_OK (0) or a negative NB_E_*
+ * code; nb_last_error() then holds a message. Device-side failures (cell pool
+ * exhausted, tree deeper than 64 levels, a bounded wait that ran out) are
+ * detected on the device and reported by the nb_step call in which they
+ * happened, where the reference only printed them under HOST_DEBUG
+ * (GPUBarnesHutNBodySimulation.java:276-290).
+ */
+#ifndef NBODY_H
+#define NBODY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define NB_OK 0
+#define NB_E_HIP (-1)   /* HIP runtime error (no device, out of memory, ...) */
+#define NB_E_POOL (-2)  /* octree cell pool exhausted (reference: buildtree.cl:112-120) */
+#define NB_E_DEPTH (-3) /* tree deeper than NB_MAX_DEPTH (reference: calculateforce.cl:69-73) */
+#define NB_E_ARG (-4)   /* invalid argument */
+#define NB_E_SYNC (-5)  /* a bounded device-side wait ran out (tree inconsistent) */
+#define NB_E_STATE (-6) /* call out of order (e.g. stage before nb_set_state) */
+#define NB_E_STACK (-7) /* force traversal stack overflow */
+#define NB_E_MODEL (-8) /* learned mode without a model, or a malformed model */
+
+#define NB_MAX_DEPTH 64 /* reference MAXDEPTH, calculateforce.cl:12 */
+
+/* ---- parameters (GPUBarnesHutNBodySimulation.java:104-108, 207-217 and the
+ *      compile-time #defines of calculateforce.cl:11-18, integrate.cl:11-12) */
+#define NB_FP64_POSITIONS 1u /* leapfrog positions kept in FP64 on the device */
+#define NB_FUSED_UPDATE 2u   /* kick/drift fused into the force kernel's epilogue */
+
+#define NB_MODE_PLAIN 0   /* plain Barnes-Hut at `theta` */
+#define NB_MODE_LEARNED 1 /* coarse traversal at `coarse_theta` + learned correction */
+
+typedef struct nb_params {
+    int32_t n_bodies;   /* NBODIES (-D NBODIES, java:212) */
+    float theta;        /* opening angle; the reference's THETA is theta^2 (calculateforce.cl:15-16) */
+    float softening;    /* the reference's EPSILON is softening^2 (calculateforce.cl:13-14); > 0 */
+    float dt;           /* TIMESTEP (calculateforce.cl:18, integrate.cl:12) */
+    int32_t vote_width; /* lanes that vote on opening a cell: 16 = the reference's
+                           work-group vote (calculateforce.cl:145, WARPSIZE java:47),
+                           32, or 64 = one wave64 */
+    uint32_t flags;     /* NB_FP64_POSITIONS | NB_FUSED_UPDATE */
+    int32_t device;     /* HIP device ordinal */
+    float pool_factor;  /* octree cell pool = max(pool_factor*N, 16384) cells */
+    int32_t mode;         /* NB_MODE_PLAIN or NB_MODE_LEARNED */
+    float coarse_theta;   /* learned mode: opening angle of the coarse traversal */
+    float check_tol;      /* learned mode: median relative error allowed on the sampled
+                             FP64 check before falling back to plain theta; 0 = no check */
+    int32_t check_every;  /* learned mode: run the sampled check every k steps (0 = never) */
+    int32_t check_samples;/* learned mode: bodies in the sampled check */
+    int32_t reserved[7];
+} nb_params;
+
+/* Defaults are the reference's shipped constants: THETA 1.5, EPSILON 0.0025,
+ * TIMESTEP 0.025, vote width 16 (calculateforce.cl:11-18). */
+void nb_default_params(nb_params *p, int32_t n_bodies);
+
+typedef struct nb_sim nb_sim;
+
+/* new GPUBarnesHutNBodySimulation(mode, nbodies, generator) + init(null):
+ * device pick, node-count sizing and buffer allocation
+ * (GPUBarnesHutNBodySimulation.java:104-151, 153-181, 219-227). Arguments are
+ * validated before any HIP call. */
+int nb_create(const nb_params *p, nb_sim **out);
+
+/* Releases the handle and its device memory (no reference counterpart: the
+ * reference leaks its OpenCL objects). */
+void nb_destroy(nb_sim *sim);
+
+/* UniverseGenerator.generate(0, n, x, y, z, vx, vy, vz, m) followed by the
+ * buffer upload of init (UniverseGenerator.java:22, GPUBarnesHutNBodySimulation.java:134-146).
+ * Host SoA arrays of n_bodies floats each. Resets the step counter to -1,
+ * accelerations to 0 and the running maximum depth to 1 (java:163-170). */
+int nb_set_state(nb_sim *sim, const float *x, const float *y, const float *z,
+                 const float *vx, const float *vy, const float *vz, const float *m);
+
+/* AbstractNBodySimulation.step() (AbstractNBodySimulation.java:16), i.e.
+ * GPUBarnesHutNBodySimulation.step() (java:249-271): bounding box, tree build,
+ * summarize, sort, force, integrate; `nsteps` times, one device error check at
+ * the end. */
+int nb_step(nb_sim *sim, int32_t nsteps);
+
+/* One stage of step() on its own (1 bbox .. 6 integrate), as the reference's
+ * executeSimulationKernel runs one kernel (java:273-291, and
+ * TestBoundingBox.java:207 runs the bounding box alone). */
+#define NB_STAGE_BBOX 1
+#define NB_STAGE_BUILD 2
+#define NB_STAGE_SUMMARIZE 3
+#define NB_STAGE_SORT 4
+#define NB_STAGE_FORCE 5
+#define NB_STAGE_INTEGRATE 6
+int nb_run_stage(nb_sim *sim, int32_t stage);
+
+/* copyVertices (copyvertices.cl:7-17, launched at java:265-266): writes
+ * n_bodies float4 (x, y, z, 1) into pos4 and (vx, vy, vz, 1) into vel4
+ * (either may be NULL). Replaces the CL-GL shared VBOs of initGLBuffers
+ * (java:229-246): the caller uploads them into its own VBOs. */
+int nb_copy_vertices(nb_sim *sim, float *pos4, float *vel4);
+
+/* getNumberOfBodies() (AbstractNBodySimulation.java:32, java:377-379). */
+int32_t nb_num_bodies(const nb_sim *sim);
+
+/* Read-back of the body state by original index (printPosition and the
+ * velocity buffer reads of java:305-374). Any pointer may be NULL. */
+int nb_get_state(nb_sim *sim, float *x, float *y, float *z, float *vx, float *vy, float *vz);
+int nb_get_accel(nb_sim *sim, float *ax, float *ay, float *az);
+int nb_get_positions64(nb_sim *sim, double *x, double *y, double *z);
+
+/* Root cell after the bounding box: out4 = (centre x, y, z, radius)
+ * (boundingbox.cl:171-189; what TestBoundingBox.java:240-246 reads). */
+int nb_get_root(nb_sim *sim, float *out4);
+
+/* sorted[] after the sort stage: body index at each sorted position
+ * (sort.cl:57-64). n_bodies int32. */
+int nb_get_sorted(nb_sim *sim, int32_t *sorted);
+
+/* Leaf membership after the build stage: per body its octant path from the
+ * root (3 bits per level, level 0 in the lowest bits, 21 levels per uint64,
+ * 3 words per body) and its depth. Replaces printChildren (java:293-303):
+ * cell numbering depends on timing, the octant path does not. */
+int nb_get_paths(nb_sim *sim, uint64_t *paths3, int32_t *depth);
+
+/* Per-cell summary after summarize, in cell order: body count, centre of mass
+ * (FP32) and mass. Arrays of at least `cap` entries; *ncells receives the
+ * number of cells written (cells that are not in the tree have count 0). */
+int nb_get_cells(nb_sim *sim, int32_t cap, int32_t *ncells, int32_t *count, float *x,
+                 float *y, float *z, float *m);
+
+typedef struct nb_counters {
+    int64_t step;          /* the reference's _step counter */
+    int64_t cells;         /* cells allocated by the last build (incl. unused spares) */
+    int64_t max_depth;     /* running maximum depth (reference _maxDepth) */
+    int64_t interactions;  /* body-node interactions of the last force stage */
+    int64_t error;         /* device error word */
+    int64_t fallbacks;     /* learned mode: steps that fell back to plain theta */
+    double stage_ms[8];    /* last step's per-stage device time (when timing enabled) */
+    double last_check_median; /* learned mode: last sampled median relative error */
+    int64_t live_cells;    /* cells linked into the tree by the last build */
+    int64_t reserved[5];
+} nb_counters;
+int nb_get_counters(nb_sim *sim, nb_counters *out);
+
+/* Per-stage HIP-event timing of the following nb_step calls (0 = off). */
+int nb_set_timing(nb_sim *sim, int32_t enabled);
+
+/* Message for the last failing call on this handle ("" if none); with a NULL
+ * handle, the message of the last nb_create that failed on this thread. */
+const char *nb_last_error(const nb_sim *sim);
+
+/* Static text for a status code. */
+const char *nb_status_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_H */

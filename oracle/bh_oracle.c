@@ -1,0 +1,711 @@
+/*
+ * bh_oracle.c -- CPU oracle for the Barnes-Hut step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * and by nothing else. The product (libnbody.so) never links, loads or calls
+ * it, and no product code path falls back to it.
+ *
+ * It restates, sequentially and in plain C, the reference pipeline of
+ * bneukom/gpu-nbody: the six kernels of kernels/nbody/<stage>.cl driven by
+ * GPUBarnesHutNBodySimulation.step() (GPUBarnesHutNBodySimulation.java:249-271).
+ * The node layout is the reference's: arrays of NN+1 entries, bodies in
+ * [0,N), cells allocated downward from the root at index NN
+ * (GPUBarnesHutNBodySimulation.java:219-227, buildtree.cl:109).
+ *
+ * Arithmetic choices where the reference leaves the rounding open (its
+ * -cl-mad-enable contraction and driver rsqrt are unpinned) are made explicit
+ * here with fmaf/fma so that the GPU kernels can reproduce them bit for bit:
+ *   - squared distance:   d2 = fmaf(dx,dx, fmaf(dy,dy, fmaf(dz,dz, eps2)))
+ *   - drift:              x  = fmaf(vh, dt, x)
+ * Centre-of-mass accumulation runs in FP64 in child-slot order (the GPU
+ * design); ORC_COM_FP32 selects the reference's FP32 arithmetic instead
+ * (summarizetree.cl:107-110,161-167). Force terms are accumulated in FP64
+ * with an FP64 inverse square root, so the oracle's accelerations carry no
+ * FP32 summation error of their own.
+ *
+ * Each function cites the reference lines it follows. How every function is
+ * pinned independently of this file (hand-derived trees, closed forms, brute
+ * force) is listed in tests/test_oracle_pins.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXDEPTH 64 /* calculateforce.cl:12 */
+#define ORC_WARPSIZE 16 /* GPUBarnesHutNBodySimulation.java:47 */
+
+/* flags */
+#define ORC_FP64_POS 1 /* leapfrog positions kept in FP64 (the GPU's precision mode) */
+#define ORC_COM_FP32 2 /* reference FP32 centre-of-mass arithmetic */
+
+/* error codes (mirrors the reference's _error word, buildtree.cl:115, calculateforce.cl:71) */
+#define ORC_OK 0
+#define ORC_E_POOL 1
+#define ORC_E_DEPTH 2
+
+typedef struct orc_sim {
+    int n, nn;
+    float theta2, eps2, dt;
+    int vote, flags;
+    float *x, *y, *z;    /* nn+1: bodies, then cell centres / centres of mass */
+    float *vx, *vy, *vz; /* n */
+    float *ax, *ay, *az; /* n */
+    float *m;            /* nn+1 */
+    double *x64, *y64, *z64; /* n, FP64 master positions (ORC_FP64_POS) */
+    double *com;         /* 4*(nn+1): FP64 centre of mass and mass of cells */
+    int *child;          /* 8*(nn+1) */
+    int *count, *start, *sorted; /* nn+1 */
+    int step, bottom, max_depth, error;
+    float radius;
+    float dq[ORC_MAXDEPTH];
+    long long interactions;
+} orc_sim;
+
+/* GPUBarnesHutNBodySimulation.java:219-227, with maxComputeUnits = 16 (:126). */
+int orc_nodes_for(int n)
+{
+    int nn = n * 2;
+    if (nn < 1024 * 16) nn = 1024 * 16;
+    while (nn & (ORC_WARPSIZE - 1)) ++nn;
+    return nn;
+}
+
+/* Opening threshold tables, calculateforce.cl:52-67: dq[0] = r^2/THETA,
+ * dq[i] = dq[i-1]/4, and EPSILON added to every entry afterwards. THETA is
+ * the square of the opening angle. The FP32 quotient is computed in double and
+ * rounded once, which equals the correctly rounded FP32 division. */
+void orc_dq_table(float radius, float theta2, float eps2, int levels, float *dq)
+{
+    float d0 = radius * radius;
+    if (theta2 > 0.0f) d0 = (float)((double)d0 / (double)theta2);
+    dq[0] = d0;
+    int i;
+    for (i = 1; i < levels; ++i) {
+        dq[i] = 0.25f * dq[i - 1];
+        dq[i - 1] += eps2;
+    }
+    dq[i - 1] += eps2;
+}
+
+orc_sim *orc_create(int n, float theta, float softening, float dt, int vote, int flags)
+{
+    if (n < 1 || vote < 1) return NULL;
+    orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+    s->n = n;
+    s->nn = orc_nodes_for(n);
+    s->theta2 = theta * theta;     /* THETA is theta squared, calculateforce.cl:15-16 */
+    s->eps2 = softening * softening; /* EPSILON is the squared softening, :13-14 */
+    s->dt = dt;
+    s->vote = vote;
+    s->flags = flags;
+    size_t nodes = (size_t)s->nn + 1;
+    s->x = calloc(nodes, sizeof(float));
+    s->y = calloc(nodes, sizeof(float));
+    s->z = calloc(nodes, sizeof(float));
+    s->m = calloc(nodes, sizeof(float));
+    s->vx = calloc(n, sizeof(float));
+    s->vy = calloc(n, sizeof(float));
+    s->vz = calloc(n, sizeof(float));
+    s->ax = calloc(n, sizeof(float));
+    s->ay = calloc(n, sizeof(float));
+    s->az = calloc(n, sizeof(float));
+    s->x64 = calloc(n, sizeof(double));
+    s->y64 = calloc(n, sizeof(double));
+    s->z64 = calloc(n, sizeof(double));
+    s->com = calloc(4 * nodes, sizeof(double));
+    s->child = malloc(8 * nodes * sizeof(int));
+    for (size_t i = 0; i < 8 * nodes; ++i) s->child[i] = -1;
+    s->count = calloc(nodes, sizeof(int));
+    s->start = calloc(nodes, sizeof(int));
+    s->sorted = calloc(nodes, sizeof(int));
+    s->step = -1;     /* GPUBarnesHutNBodySimulation.java:167 */
+    s->max_depth = 1; /* GPUBarnesHutNBodySimulation.java:170 */
+    return s;
+}
+
+void orc_destroy(orc_sim *s)
+{
+    if (!s) return;
+    free(s->x); free(s->y); free(s->z); free(s->m);
+    free(s->vx); free(s->vy); free(s->vz);
+    free(s->ax); free(s->ay); free(s->az);
+    free(s->x64); free(s->y64); free(s->z64);
+    free(s->com); free(s->child); free(s->count); free(s->start); free(s->sorted);
+    free(s);
+}
+
+/* Initial state, as UniverseGenerator.generate fills the host arrays that
+ * GPUBarnesHutNBodySimulation.init copies to the device (java:134-146).
+ * Accelerations start at zero (java:163-165). */
+void orc_set_state(orc_sim *s, const float *x, const float *y, const float *z,
+                   const float *vx, const float *vy, const float *vz, const float *m)
+{
+    int n = s->n;
+    memcpy(s->x, x, n * sizeof(float));
+    memcpy(s->y, y, n * sizeof(float));
+    memcpy(s->z, z, n * sizeof(float));
+    memcpy(s->vx, vx, n * sizeof(float));
+    memcpy(s->vy, vy, n * sizeof(float));
+    memcpy(s->vz, vz, n * sizeof(float));
+    memcpy(s->m, m, n * sizeof(float));
+    memset(s->ax, 0, n * sizeof(float));
+    memset(s->ay, 0, n * sizeof(float));
+    memset(s->az, 0, n * sizeof(float));
+    for (int i = 0; i < n; ++i) {
+        s->x64[i] = x[i];
+        s->y64[i] = y[i];
+        s->z64[i] = z[i];
+    }
+}
+
+/* Overwrite accelerations and the step counter (lets a test hand the oracle a
+ * GPU state in the middle of a run). */
+void orc_set_accel(orc_sim *s, const float *ax, const float *ay, const float *az, int step)
+{
+    memcpy(s->ax, ax, s->n * sizeof(float));
+    memcpy(s->ay, ay, s->n * sizeof(float));
+    memcpy(s->az, az, s->n * sizeof(float));
+    s->step = step;
+}
+
+void orc_set_positions64(orc_sim *s, const double *x, const double *y, const double *z)
+{
+    memcpy(s->x64, x, s->n * sizeof(double));
+    memcpy(s->y64, y, s->n * sizeof(double));
+    memcpy(s->z64, z, s->n * sizeof(double));
+    for (int i = 0; i < s->n; ++i) {
+        s->x[i] = (float)x[i];
+        s->y[i] = (float)y[i];
+        s->z[i] = (float)z[i];
+    }
+}
+
+/* ---- stage 1: bounding box, boundingbox.cl:43-92 (scan), :152-160 (combine),
+ *      :171-195 (root cell and counters) ---------------------------------- */
+void orc_bbox_arrays(int n, const float *x, const float *y, const float *z, float *out7)
+{
+    float mnx = x[0], mny = y[0], mnz = z[0]; /* boundingbox.cl:49-58 */
+    float mxx = x[0], mxy = y[0], mxz = z[0];
+    for (int i = 0; i < n; ++i) {
+        mnx = fminf(mnx, x[i]); mxx = fmaxf(mxx, x[i]);
+        mny = fminf(mny, y[i]); mxy = fmaxf(mxy, y[i]);
+        mnz = fminf(mnz, z[i]); mxz = fmaxf(mxz, z[i]);
+    }
+    out7[0] = 0.5f * (mnx + mxx); /* boundingbox.cl:171-173 */
+    out7[1] = 0.5f * (mny + mxy);
+    out7[2] = 0.5f * (mnz + mxz);
+    out7[3] = 0.5f * fmaxf(fmaxf(mxx - mnx, mxy - mny), mxz - mnz); /* :179 */
+    out7[4] = mnx; out7[5] = mny; out7[6] = mnz;
+}
+
+void orc_bbox(orc_sim *s)
+{
+    float r[7];
+    orc_bbox_arrays(s->n, s->x, s->y, s->z, r);
+    int nn = s->nn;
+    s->radius = r[3];
+    s->bottom = nn;          /* :180 */
+    s->x[nn] = r[0];         /* :185-187 */
+    s->y[nn] = r[1];
+    s->z[nn] = r[2];
+    s->m[nn] = -1.0f;        /* :188 */
+    s->start[nn] = 0;        /* :189 */
+    for (int i = 0; i < 8; ++i) s->child[8 * nn + i] = -1; /* :192-193 */
+    s->step++;               /* :195 */
+}
+
+/* ---- stage 2: tree build, buildtree.cl:41-192, one body at a time -------- */
+static inline int octant_of(float cx, float cy, float cz, float bx, float by, float bz)
+{
+    /* buildtree.cl:66-68 and :84-86: strict '<', ties go to the low side */
+    int p = 0;
+    if (cx < bx) p = 1;
+    if (cy < by) p += 2;
+    if (cz < bz) p += 4;
+    return p;
+}
+
+int orc_build(orc_sim *s)
+{
+    const int n = s->n, nn = s->nn;
+    float *X = s->x, *Y = s->y, *Z = s->z;
+    int *child = s->child;
+    const float radius = s->radius;
+    int local_max = 1;
+    for (int b = 0; b < n; ++b) {
+        const float bx = X[b], by = Y[b], bz = Z[b];
+        int node = nn, depth = 1; /* buildtree.cl:60-62 */
+        float r = radius;
+        int path = octant_of(X[nn], Y[nn], Z[nn], bx, by, bz);
+        int ci = child[8 * node + path];
+        while (ci >= n) { /* :77-89 */
+            node = ci;
+            ++depth;
+            r *= 0.5f;
+            path = octant_of(X[node], Y[node], Z[node], bx, by, bz);
+            ci = child[8 * node + path];
+        }
+        if (ci == -1) { /* :98-101 */
+            child[8 * node + path] = b;
+        } else { /* :102-181 */
+            const int locked = 8 * node + path;
+            int patch = -1;
+            do {
+                depth++;
+                const int cell = --s->bottom; /* atom_dec(_bottom) - 1, :109 */
+                if (cell <= n) { /* :112-120 */
+                    s->error = ORC_E_POOL;
+                    s->bottom = nn;
+                    return ORC_E_POOL;
+                }
+                if (cell > patch) patch = cell;
+                float xo = (float)(path & 1) * r; /* :124-126 */
+                float yo = (float)((path >> 1) & 1) * r;
+                float zo = (float)((path >> 2) & 1) * r;
+                r *= 0.5f; /* :128 */
+                s->m[cell] = -1.0f; /* :131-132 */
+                s->start[cell] = -1;
+                const float cx = (X[node] - r) + xo; /* :134-136 */
+                const float cy = (Y[node] - r) + yo;
+                const float cz = (Z[node] - r) + zo;
+                X[cell] = cx; Y[cell] = cy; Z[cell] = cz;
+                for (int k = 0; k < 8; ++k) child[8 * cell + k] = -1; /* :139 */
+                if (patch != cell) child[8 * node + path] = cell; /* :141-146 */
+                /* re-insert the old body, :148-152 */
+                child[8 * cell + octant_of(cx, cy, cz, X[ci], Y[ci], Z[ci])] = ci;
+                node = cell; /* :155-161 */
+                path = octant_of(cx, cy, cz, bx, by, bz);
+                ci = child[8 * node + path];
+            } while (ci >= 0);
+            child[8 * node + path] = b; /* :169 */
+            child[locked] = patch;      /* :180 */
+        }
+        if (depth > local_max) local_max = depth; /* :186 */
+    }
+    if (local_max > s->max_depth) s->max_depth = local_max; /* atom_max, :199 */
+    return ORC_OK;
+}
+
+/* ---- stage 3: summarize, summarizetree.cl:55-178, cells in ascending index
+ * order (children always have lower indices than their parent, since cells
+ * are allocated downward), children compacted to the front (:75-81). ----- */
+void orc_summarize(orc_sim *s)
+{
+    const int n = s->n, nn = s->nn;
+    float *X = s->x, *Y = s->y, *Z = s->z, *M = s->m;
+    int *child = s->child;
+    const int fp32 = (s->flags & ORC_COM_FP32) != 0;
+    for (int node = s->bottom; node <= nn; ++node) {
+        int used = 0, cnt = 0;
+        double m64 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        float m32 = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            const int c = child[8 * node + k];
+            if (c < 0) continue;
+            if (k != used) { /* move to front, :77-81 */
+                child[8 * node + k] = -1;
+                child[8 * node + used] = c;
+            }
+            ++used;
+            if (c >= n) {
+                cnt += s->count[c]; /* :98-105 with :118 */
+                if (fp32) {
+                    const float mc = M[c];
+                    m32 += mc;
+                    fx = fmaf(X[c], mc, fx); fy = fmaf(Y[c], mc, fy); fz = fmaf(Z[c], mc, fz);
+                } else {
+                    const double *cc = s->com + 4 * (size_t)c;
+                    m64 += cc[3];
+                    sx = fma(cc[3], cc[0], sx); sy = fma(cc[3], cc[1], sy); sz = fma(cc[3], cc[2], sz);
+                }
+            } else {
+                cnt += 1;
+                const float mc = M[c];
+                if (fp32) {
+                    m32 += mc;
+                    fx = fmaf(X[c], mc, fx); fy = fmaf(Y[c], mc, fy); fz = fmaf(Z[c], mc, fz);
+                } else {
+                    m64 += (double)mc;
+                    sx = fma((double)mc, (double)X[c], sx);
+                    sy = fma((double)mc, (double)Y[c], sy);
+                    sz = fma((double)mc, (double)Z[c], sz);
+                }
+            }
+        }
+        s->count[node] = cnt; /* :160 */
+        double *cc = s->com + 4 * (size_t)node;
+        if (fp32) {
+            /* :161-167; a cell of zero mass gets its centre at the origin
+             * instead of the reference's 0*inf = NaN */
+            const float inv = m32 > 0.0f ? 1.0f / m32 : 0.0f;
+            X[node] = fx * inv; Y[node] = fy * inv; Z[node] = fz * inv;
+            M[node] = m32;
+            cc[0] = X[node]; cc[1] = Y[node]; cc[2] = Z[node]; cc[3] = m32;
+        } else {
+            const double inv = m64 > 0.0 ? 1.0 / m64 : 0.0;
+            cc[0] = sx * inv; cc[1] = sy * inv; cc[2] = sz * inv; cc[3] = m64;
+            X[node] = (float)cc[0]; Y[node] = (float)cc[1]; Z[node] = (float)cc[2];
+            M[node] = (float)m64; /* the ready flag of :172, as a value */
+        }
+    }
+}
+
+/* ---- stage 4: sort, sort.cl:24-75: top-down start offsets; parents have
+ * higher indices than their children, so descending order sees every
+ * parent before its children. ------------------------------------------- */
+void orc_sort(orc_sim *s)
+{
+    const int n = s->n, nn = s->nn;
+    for (int cell = nn; cell >= s->bottom; --cell) {
+        int st = s->start[cell];
+        for (int i = 0; i < 8; ++i) {
+            const int ch = s->child[8 * cell + i];
+            if (ch >= n) { /* :41-56 */
+                s->start[ch] = st;
+                st += s->count[ch];
+            } else if (ch >= 0) { /* :57-64 */
+                s->sorted[st] = ch;
+                ++st;
+            }
+        }
+    }
+}
+
+/* ---- stage 5: force, calculateforce.cl:52-186. Vote groups are `vote`
+ * consecutive sorted positions; a cell is accepted only if every member of
+ * the group accepts it (work_group_all, :145), bodies always interact. --- */
+static void force_group(orc_sim *s, int g, long long *inter)
+{
+    const int n = s->n, nn = s->nn, W = s->vote;
+    const float *X = s->x, *Y = s->y, *Z = s->z, *M = s->m;
+    const int *child = s->child;
+    const float eps2 = s->eps2;
+    const int first = g * W;
+    const int cnt = (first + W <= n) ? W : n - first;
+    float px[256], py[256], pz[256];
+    double ax[256], ay[256], az[256];
+    int stackNode[ORC_MAXDEPTH + 1], stackPos[ORC_MAXDEPTH + 1];
+    for (int l = 0; l < cnt; ++l) {
+        const int b = s->sorted[first + l]; /* :101 */
+        px[l] = X[b]; py[l] = Y[b]; pz[l] = Z[b];
+        ax[l] = ay[l] = az[l] = 0.0;
+    }
+    long long ni = 0;
+    int depth = 0; /* :112-116 */
+    stackNode[0] = nn;
+    stackPos[0] = 0;
+    while (depth >= 0) { /* :122 */
+        int top;
+        while ((top = stackPos[depth]) < 8) { /* :125 */
+            const int c = child[stackNode[depth] * 8 + top];
+            stackPos[depth] = top + 1;
+            if (c >= 0) {
+                const float cxn = X[c], cyn = Y[c], czn = Z[c];
+                int accept = 1;
+                if (c >= n) { /* work_group_all(d2 >= dq[depth]), :145 */
+                    for (int l = 0; l < cnt && accept; ++l) {
+                        const float dx = cxn - px[l], dy = cyn - py[l], dz = czn - pz[l];
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                        if (!(d2 >= s->dq[depth])) accept = 0;
+                    }
+                }
+                if (accept) { /* :146-151 */
+                    const double mc = (double)M[c];
+                    for (int l = 0; l < cnt; ++l) {
+                        const float dx = cxn - px[l], dy = cyn - py[l], dz = czn - pz[l];
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                        const double rinv = 1.0 / sqrt((double)d2);
+                        const double f = mc * rinv * rinv * rinv;
+                        ax[l] += (double)dx * f;
+                        ay[l] += (double)dy * f;
+                        az[l] += (double)dz * f;
+                    }
+                    ni += cnt;
+                } else { /* push, :155-160 */
+                    depth++;
+                    stackNode[depth] = c;
+                    stackPos[depth] = 0;
+                }
+            } else {
+                depth = depth - 1 > 0 ? depth - 1 : 0; /* :165-167 */
+            }
+        }
+        depth--; /* :171 */
+    }
+    const float dt = s->dt;
+    for (int l = 0; l < cnt; ++l) {
+        const int b = s->sorted[first + l];
+        const float fx = (float)ax[l], fy = (float)ay[l], fz = (float)az[l];
+        if (s->step > 0) { /* :174-179 */
+            s->vx[b] = s->vx[b] + ((fx - s->ax[b]) * dt) * 0.5f;
+            s->vy[b] = s->vy[b] + ((fy - s->ay[b]) * dt) * 0.5f;
+            s->vz[b] = s->vz[b] + ((fz - s->az[b]) * dt) * 0.5f;
+        }
+        s->ax[b] = fx; /* :183-185 */
+        s->ay[b] = fy;
+        s->az[b] = fz;
+    }
+    *inter += ni;
+}
+
+int orc_force(orc_sim *s)
+{
+    orc_dq_table(s->radius, s->theta2, s->eps2, s->max_depth, s->dq); /* :52-67 */
+    if (s->max_depth > ORC_MAXDEPTH) { /* :69-73 */
+        s->error = ORC_E_DEPTH;
+        return ORC_E_DEPTH;
+    }
+    if (s->vote > 256) return -1;
+    const int groups = (s->n + s->vote - 1) / s->vote;
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+    for (int g = 0; g < groups; ++g) force_group(s, g, &total);
+    s->interactions = total;
+    return ORC_OK;
+}
+
+/* Force on a subset of vote groups only (for sampled checks at large N).
+ * Accelerations of bodies outside the listed groups are left untouched. */
+int orc_force_groups(orc_sim *s, const int *groups, int ngroups)
+{
+    orc_dq_table(s->radius, s->theta2, s->eps2, s->max_depth, s->dq);
+    if (s->max_depth > ORC_MAXDEPTH) {
+        s->error = ORC_E_DEPTH;
+        return ORC_E_DEPTH;
+    }
+    if (s->vote > 256) return -1;
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int i = 0; i < ngroups; ++i) force_group(s, groups[i], &total);
+    s->interactions = total;
+    return ORC_OK;
+}
+
+/* ---- stage 6: integrate, integrate.cl:27-44 (kick, drift, kick with the
+ * acceleration just computed). ------------------------------------------ */
+void orc_integrate(orc_sim *s)
+{
+    const float dt = s->dt;
+    const int fp64 = (s->flags & ORC_FP64_POS) != 0;
+    for (int i = 0; i < s->n; ++i) {
+        const float dvx = (s->ax[i] * dt) * 0.5f; /* :29-31 */
+        const float dvy = (s->ay[i] * dt) * 0.5f;
+        const float dvz = (s->az[i] * dt) * 0.5f;
+        const float vhx = s->vx[i] + dvx; /* :33-35 */
+        const float vhy = s->vy[i] + dvy;
+        const float vhz = s->vz[i] + dvz;
+        if (fp64) {
+            s->x64[i] = fma((double)vhx, (double)dt, s->x64[i]);
+            s->y64[i] = fma((double)vhy, (double)dt, s->y64[i]);
+            s->z64[i] = fma((double)vhz, (double)dt, s->z64[i]);
+            s->x[i] = (float)s->x64[i];
+            s->y[i] = (float)s->y64[i];
+            s->z[i] = (float)s->z64[i];
+        } else {
+            s->x[i] = fmaf(vhx, dt, s->x[i]); /* :37-39 */
+            s->y[i] = fmaf(vhy, dt, s->y[i]);
+            s->z[i] = fmaf(vhz, dt, s->z[i]);
+        }
+        s->vx[i] = vhx + dvx; /* :41-43 */
+        s->vy[i] = vhy + dvy;
+        s->vz[i] = vhz + dvz;
+    }
+}
+
+/* ---- step(): GPUBarnesHutNBodySimulation.java:249-271 --------------------- */
+int orc_step(orc_sim *s, int nsteps)
+{
+    for (int k = 0; k < nsteps; ++k) {
+        orc_bbox(s);
+        int e = orc_build(s);
+        if (e) return e;
+        orc_summarize(s);
+        orc_sort(s);
+        e = orc_force(s);
+        if (e) return e;
+        orc_integrate(s);
+    }
+    return ORC_OK;
+}
+
+/* Run only stages [first, last] of one step (1 = bbox ... 6 = integrate),
+ * the way GPUBarnesHutNBodySimulation.executeSimulationKernel (java:273-291)
+ * lets a caller drive them one by one. */
+int orc_run_stages(orc_sim *s, int first, int last)
+{
+    for (int st = first; st <= last; ++st) {
+        int e = 0;
+        switch (st) {
+        case 1: orc_bbox(s); break;
+        case 2: e = orc_build(s); break;
+        case 3: orc_summarize(s); break;
+        case 4: orc_sort(s); break;
+        case 5: e = orc_force(s); break;
+        case 6: orc_integrate(s); break;
+        default: return -1;
+        }
+        if (e) return e;
+    }
+    return ORC_OK;
+}
+
+/* ---- read-back helpers ---------------------------------------------------- */
+void orc_get_state(const orc_sim *s, float *x, float *y, float *z, float *vx, float *vy, float *vz)
+{
+    const int n = s->n;
+    memcpy(x, s->x, n * sizeof(float));
+    memcpy(y, s->y, n * sizeof(float));
+    memcpy(z, s->z, n * sizeof(float));
+    memcpy(vx, s->vx, n * sizeof(float));
+    memcpy(vy, s->vy, n * sizeof(float));
+    memcpy(vz, s->vz, n * sizeof(float));
+}
+
+void orc_get_accel(const orc_sim *s, float *ax, float *ay, float *az)
+{
+    memcpy(ax, s->ax, s->n * sizeof(float));
+    memcpy(ay, s->ay, s->n * sizeof(float));
+    memcpy(az, s->az, s->n * sizeof(float));
+}
+
+void orc_get_positions64(const orc_sim *s, double *x, double *y, double *z)
+{
+    memcpy(x, s->x64, s->n * sizeof(double));
+    memcpy(y, s->y64, s->n * sizeof(double));
+    memcpy(z, s->z64, s->n * sizeof(double));
+}
+
+void orc_get_sorted(const orc_sim *s, int *sorted)
+{
+    memcpy(sorted, s->sorted, s->n * sizeof(int));
+}
+
+/* scalars: [0]=step [1]=bottom [2]=max_depth [3]=error [4]=cells [5]=nn */
+void orc_get_scalars(const orc_sim *s, long long *out)
+{
+    out[0] = s->step;
+    out[1] = s->bottom;
+    out[2] = s->max_depth;
+    out[3] = s->error;
+    out[4] = s->nn - s->bottom + 1;
+    out[5] = s->nn;
+    out[6] = s->interactions;
+}
+
+/* root centre and radius after stage 1 */
+void orc_get_root(const orc_sim *s, float *out4)
+{
+    out4[0] = s->x[s->nn];
+    out4[1] = s->y[s->nn];
+    out4[2] = s->z[s->nn];
+    out4[3] = s->radius;
+}
+
+/* Leaf membership: for every body its octant path from the root (3 bits per
+ * level, level 0 in the lowest bits of paths[3*i], 21 levels per word) and its
+ * depth (number of octants on the path). Cell numbering is timing-dependent in
+ * the reference; the path is not. The slot index is the octant only before
+ * summarize compacts the child slots, so paths are read after stage 2.
+ * Returns the number of bodies reached, or -1 on a tree deeper than 63. */
+static void put_path(unsigned long long *dst, const unsigned long long *cur, int levels)
+{
+    for (int w = 0; w < 3; ++w) {
+        const int lv = levels - 21 * w;
+        if (lv <= 0) dst[w] = 0;
+        else if (lv >= 21) dst[w] = cur[w] & ((1ULL << 63) - 1);
+        else dst[w] = cur[w] & ((1ULL << (3 * lv)) - 1);
+    }
+}
+
+int orc_get_paths(const orc_sim *s, unsigned long long *paths, int *depth)
+{
+    const int n = s->n, nn = s->nn;
+    int reached = 0;
+    int stackNode[ORC_MAXDEPTH + 2], stackPos[ORC_MAXDEPTH + 2];
+    unsigned long long cur[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) depth[i] = 0;
+    int d = 0;
+    stackNode[0] = nn;
+    stackPos[0] = 0;
+    while (d >= 0) {
+        if (stackPos[d] >= 8) { --d; continue; }
+        const int k = stackPos[d]++;
+        const int c = s->child[8 * stackNode[d] + k];
+        if (c < 0) continue;
+        const int w = d / 21, sh = 3 * (d % 21);
+        cur[w] = (cur[w] & ~(7ULL << sh)) | ((unsigned long long)k << sh);
+        if (c < n) {
+            put_path(paths + 3 * (size_t)c, cur, d + 1);
+            depth[c] = d + 1;
+            ++reached;
+        } else {
+            if (d + 1 >= 63) return -1;
+            ++d;
+            stackNode[d] = c;
+            stackPos[d] = 0;
+        }
+    }
+    return reached;
+}
+
+/* Per-cell data in the reference layout for [bottom, nn]: count, FP32 centre
+ * of mass and mass. out must hold (nn-bottom+1) entries each. */
+void orc_get_cells(const orc_sim *s, int *count, float *x, float *y, float *z, float *m, int *child8)
+{
+    const int b = s->bottom, nn = s->nn;
+    for (int c = b; c <= nn; ++c) {
+        count[c - b] = s->count[c];
+        x[c - b] = s->x[c];
+        y[c - b] = s->y[c];
+        z[c - b] = s->z[c];
+        m[c - b] = s->m[c];
+        for (int k = 0; k < 8; ++k) child8[8 * (c - b) + k] = s->child[8 * c + k];
+    }
+}
+
+/* ---- FP64 direct sum on sampled targets (the error yardstick) ------------ */
+void orc_direct_fp64(int n, const float *x, const float *y, const float *z, const float *m,
+                     double eps2, int ns, const int *idx, double *out_ax, double *out_ay, double *out_az)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < ns; ++t) {
+        const int i = idx[t];
+        const double xi = x[i], yi = y[i], zi = z[i];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double dx = (double)x[j] - xi, dy = (double)y[j] - yi, dz = (double)z[j] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+            const double r = 1.0 / sqrt(d2);
+            const double f = (double)m[j] * r * r * r;
+            sx += dx * f;
+            sy += dy * f;
+            sz += dz * f;
+        }
+        out_ax[t] = sx;
+        out_ay[t] = sy;
+        out_az[t] = sz;
+    }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int t)
+{
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}

@@ -1,0 +1,202 @@
+"""ctypes binding of the CPU oracle (oracle/bh_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
+The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+FP64_POS = 1
+COM_FP32 = 2
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.orc_nodes_for.argtypes = [C.c_int]
+        L.orc_nodes_for.restype = C.c_int
+        L.orc_create.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_state.argtypes = [vp] + [_f32p] * 7
+        L.orc_set_accel.argtypes = [vp, _f32p, _f32p, _f32p, C.c_int]
+        L.orc_set_positions64.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.orc_bbox_arrays.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p]
+        L.orc_dq_table.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, _f32p]
+        for name in ("orc_bbox", "orc_summarize", "orc_sort", "orc_integrate"):
+            getattr(L, name).argtypes = [vp]
+        for name in ("orc_build", "orc_force"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = C.c_int
+        L.orc_force_groups.argtypes = [vp, _i32p, C.c_int]
+        L.orc_force_groups.restype = C.c_int
+        L.orc_step.argtypes = [vp, C.c_int]
+        L.orc_step.restype = C.c_int
+        L.orc_run_stages.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_run_stages.restype = C.c_int
+        L.orc_get_state.argtypes = [vp] + [_f32p] * 6
+        L.orc_get_accel.argtypes = [vp] + [_f32p] * 3
+        L.orc_get_positions64.argtypes = [vp] + [_f64p] * 3
+        L.orc_get_sorted.argtypes = [vp, _i32p]
+        L.orc_get_scalars.argtypes = [vp, _i64p]
+        L.orc_get_root.argtypes = [vp, _f32p]
+        L.orc_get_paths.argtypes = [vp, _u64p, _i32p]
+        L.orc_get_paths.restype = C.c_int
+        L.orc_get_cells.argtypes = [vp, _i32p, _f32p, _f32p, _f32p, _f32p, _i32p]
+        L.orc_direct_fp64.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int,
+                                      _i32p, _f64p, _f64p, _f64p]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def nodes_for(n: int) -> int:
+    return lib().orc_nodes_for(n)
+
+
+def bbox(x, y, z):
+    """(root_x, root_y, root_z, radius, min_x, min_y, min_z) per boundingbox.cl."""
+    out = np.zeros(7, np.float32)
+    lib().orc_bbox_arrays(len(x), _f32(x), _f32(y), _f32(z), out)
+    return out
+
+
+def dq_table(radius, theta, softening, levels):
+    out = np.zeros(max(levels, 1), np.float32)
+    t2 = np.float32(theta) * np.float32(theta)
+    e2 = np.float32(softening) * np.float32(softening)
+    lib().orc_dq_table(float(radius), float(t2), float(e2), levels, out)
+    return out
+
+
+def direct_fp64(x, y, z, m, softening, idx):
+    """FP64 direct-sum accelerations (G=1) on the bodies listed in idx."""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    ax = np.zeros(len(idx)); ay = np.zeros(len(idx)); az = np.zeros(len(idx))
+    lib().orc_direct_fp64(len(x), _f32(x), _f32(y), _f32(z), _f32(m),
+                          float(softening) ** 2 if softening >= 0 else 0.0,
+                          len(idx), idx, ax, ay, az)
+    return np.stack([ax, ay, az], axis=1)
+
+
+class Oracle:
+    """One reference-semantics simulation on the CPU (mirrors the engine API)."""
+
+    def __init__(self, n, theta=0.5, softening=0.05, dt=0.025, vote=16, flags=0):
+        self.n = n
+        self.h = lib().orc_create(n, theta, softening, dt, vote, flags)
+        if not self.h:
+            raise ValueError("orc_create rejected the parameters")
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, x, y, z, vx, vy, vz, m):
+        lib().orc_set_state(self.h, _f32(x), _f32(y), _f32(z), _f32(vx), _f32(vy), _f32(vz), _f32(m))
+
+    def set_accel(self, ax, ay, az, step):
+        lib().orc_set_accel(self.h, _f32(ax), _f32(ay), _f32(az), int(step))
+
+    def set_positions64(self, x, y, z):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        lib().orc_set_positions64(self.h, f(x), f(y), f(z))
+
+    def step(self, k=1):
+        return lib().orc_step(self.h, k)
+
+    def run_stages(self, first, last):
+        return lib().orc_run_stages(self.h, first, last)
+
+    def force_groups(self, groups):
+        g = np.ascontiguousarray(groups, dtype=np.int32)
+        return lib().orc_force_groups(self.h, g, len(g))
+
+    def state(self):
+        arrs = [np.zeros(self.n, np.float32) for _ in range(6)]
+        lib().orc_get_state(self.h, *arrs)
+        return arrs
+
+    def accel(self):
+        arrs = [np.zeros(self.n, np.float32) for _ in range(3)]
+        lib().orc_get_accel(self.h, *arrs)
+        return np.stack(arrs, axis=1)
+
+    def positions64(self):
+        arrs = [np.zeros(self.n, np.float64) for _ in range(3)]
+        lib().orc_get_positions64(self.h, *arrs)
+        return np.stack(arrs, axis=1)
+
+    def sorted(self):
+        out = np.zeros(self.n, np.int32)
+        lib().orc_get_sorted(self.h, out)
+        return out
+
+    def scalars(self):
+        out = np.zeros(8, np.int64)
+        lib().orc_get_scalars(self.h, out)
+        keys = ("step", "bottom", "max_depth", "error", "cells", "nn", "interactions")
+        return dict(zip(keys, (int(v) for v in out[:7])))
+
+    def root(self):
+        out = np.zeros(4, np.float32)
+        lib().orc_get_root(self.h, out)
+        return out
+
+    def paths(self):
+        """(paths uint64[n,3], depth int32[n]); valid after stage 2."""
+        p = np.zeros((self.n, 3), np.uint64)
+        d = np.zeros(self.n, np.int32)
+        r = lib().orc_get_paths(self.h, p.reshape(-1), d)
+        if r != self.n:
+            raise RuntimeError(f"oracle tree reaches {r} of {self.n} bodies")
+        return p, d
+
+    def cells(self):
+        sc = self.scalars()
+        c = sc["cells"]
+        cnt = np.zeros(c, np.int32)
+        x = np.zeros(c, np.float32); y = np.zeros(c, np.float32); z = np.zeros(c, np.float32)
+        m = np.zeros(c, np.float32)
+        ch = np.zeros(8 * c, np.int32)
+        lib().orc_get_cells(self.h, cnt, x, y, z, m, ch)
+        return dict(count=cnt, x=x, y=y, z=z, m=m, child=ch.reshape(c, 8))

@@ -1,0 +1,41 @@
+"""Shared helpers for the parity tests: run the engine and the oracle on the
+same seeded input and compare what each stage produced."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def make_sim(state, **kw):
+    """Engine handle with `state` uploaded (accelerations 0, step counter -1)."""
+    from nbody_amd import HipBarnesHutNBodySimulation, Mode
+    from nbody_amd.universe import ArrayUniverse
+
+    n = len(state[0])
+    sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, n, ArrayUniverse(state), **kw)
+    sim.init(None)
+    return sim
+
+
+def make_oracle(state, theta, softening, dt, vote, flags=0):
+    import pyoracle
+
+    o = pyoracle.Oracle(len(state[0]), theta=theta, softening=softening, dt=dt, vote=vote, flags=flags)
+    o.set_state(*state)
+    return o
+
+
+def rel_err(a, ref):
+    """Per-body |a - ref| / |ref| (vector norms)."""
+    num = np.linalg.norm(np.asarray(a, np.float64) - np.asarray(ref, np.float64), axis=1)
+    den = np.linalg.norm(np.asarray(ref, np.float64), axis=1)
+    return num / np.maximum(den, 1e-300)
+
+
+def cell_multiset(count, x, y, z, m):
+    """Live cells (count > 0) as a lexicographically sorted structured array,
+    so trees with different cell numbering compare equal bit for bit."""
+    keep = count > 0
+    rec = np.zeros(int(keep.sum()), dtype=[("count", "<i4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"),
+                                            ("m", "<f4")])
+    rec["count"], rec["x"], rec["y"], rec["z"], rec["m"] = count[keep], x[keep], y[keep], z[keep], m[keep]
+    return np.sort(rec, order=["count", "x", "y", "z", "m"])
