@@ -23,7 +23,7 @@ struct nb_sim {
     int cus = 0;
     hipStream_t stream = nullptr;
     nb::Dev d{};
-    int bbox_blocks = 1, build_blocks = 1, summarize_blocks = 1, sort_blocks = 1;
+    int bbox_blocks = 1, build_blocks = 1;
     bool have_state = false;
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -32,6 +32,8 @@ struct nb_sim {
     long long last_interactions = 0;
     long long fallbacks = 0;
     double last_check_median = 0.0;
+    int levels = 0;                      // non-empty cell levels of the current tree
+    int lvl_start[nb::kMaxDepth + 2] = {}; // host copy of the level list offsets
     std::string err;
 };
 
@@ -104,7 +106,6 @@ nb::Ctl initial_ctl(const nb_sim *s, uint32_t epoch)
     c.error = 0;
     c.bbox_count = 0;
     c.interactions = 0;
-    c.spin_limit = 1 << 20;
     (void)s;
     return c;
 }
@@ -133,6 +134,20 @@ int check_device_error(nb_sim *s)
     }
 }
 
+// Groups the live cells by level and reads the level sizes back to the host
+// (the only host synchronization inside a step).
+int prepare_levels(nb_sim *s)
+{
+    hipError_t e = nb::launch_levels(s->d, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "level grouping launch: %s", hipGetErrorString(e));
+    nb::Ctl c;
+    int r = read_ctl(s, &c);
+    if (r) return r;
+    s->levels = c.error ? 0 : c.levels;
+    std::memcpy(s->lvl_start, c.lvl_start, sizeof s->lvl_start);
+    return NB_OK;
+}
+
 int launch_stage(nb_sim *s, int stage)
 {
     const nb::Dev &d = s->d;
@@ -140,9 +155,21 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD: e = nb::launch_build(d, s->build_blocks, s->stream); break;
-    case NB_STAGE_SUMMARIZE: e = nb::launch_summarize(d, s->summarize_blocks, s->stream); break;
-    case NB_STAGE_SORT: e = nb::launch_sort(d, s->sort_blocks, s->stream); break;
+    case NB_STAGE_BUILD:
+        e = nb::launch_build(d, s->build_blocks, s->stream);
+        s->levels = 0;
+        break;
+    case NB_STAGE_SUMMARIZE: {
+        int r = prepare_levels(s);
+        if (r) return r;
+        for (int L = s->levels - 1; L >= 0 && e == hipSuccess; --L)
+            e = nb::launch_summarize_level(d, L, s->lvl_start[L + 1] - s->lvl_start[L], s->stream);
+        break;
+    }
+    case NB_STAGE_SORT:
+        for (int L = 0; L < s->levels && e == hipSuccess; ++L)
+            e = nb::launch_sort_level(d, L, s->lvl_start[L + 1] - s->lvl_start[L], s->stream);
+        break;
     case NB_STAGE_FORCE: e = nb::launch_force(d, s->p.vote_width, fused, s->stream); break;
     case NB_STAGE_INTEGRATE:
         if (!fused) e = nb::launch_integrate(d, s->stream);
@@ -156,15 +183,22 @@ int launch_stage(nb_sim *s, int stage)
 void free_all(nb_sim *s)
 {
     nb::Dev &d = s->d;
-    void *ptrs[] = {d.ctl,   d.x,     d.y,      d.z,     d.vx,    d.vy,    d.vz,     d.ax,
-                    d.ay,    d.az,    d.m,      d.x64,   d.y64,   d.z64,   d.slot,   d.ready,
-                    d.sready, d.count, d.start, d.com64, d.cnode, d.cchild, d.sorted, d.bnode,
-                    d.bbox_part};
+    void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
+                    d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
+                    d.lvl_blockhist,  d.nodes4, d.sorted,        d.bbox_part};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+template <class T> int fetch(nb_sim *s, std::vector<T> &out, const T *src, size_t count)
+{
+    out.resize(count);
+    HIP_TRY(s, hipMemcpyAsync(out.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
 }
 
 } // namespace
@@ -228,25 +262,25 @@ int nb_create(const nb_params *p, nb_sim **out)
     d.fp64 = (p->flags & NB_FP64_POSITIONS) ? 1 : 0;
     const size_t cap = (size_t)d.cap;
     bool ok = dalloc(&d.ctl, 1) == hipSuccess;
-    for (float **f : {&d.x, &d.y, &d.z, &d.vx, &d.vy, &d.vz, &d.ax, &d.ay, &d.az, &d.m})
-        ok = ok && dalloc(f, (size_t)n) == hipSuccess;
-    if (d.fp64)
-        for (double **f : {&d.x64, &d.y64, &d.z64}) ok = ok && dalloc(f, (size_t)n) == hipSuccess;
+    for (float4 **f : {&d.pm4, &d.vel4, &d.acc4}) ok = ok && dalloc(f, (size_t)n) == hipSuccess;
+    if (d.fp64) ok = ok && dalloc(&d.pos64, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.slot, 8 * cap) == hipSuccess;
-    ok = ok && dalloc(&d.ready, cap) == hipSuccess && dalloc(&d.sready, cap) == hipSuccess;
+    ok = ok && dalloc(&d.cinfo, cap) == hipSuccess;
     ok = ok && dalloc(&d.count, cap) == hipSuccess && dalloc(&d.start, cap) == hipSuccess;
-    ok = ok && dalloc(&d.com64, cap) == hipSuccess && dalloc(&d.cnode, cap) == hipSuccess;
+    ok = ok && dalloc(&d.com64, cap) == hipSuccess;
     ok = ok && dalloc(&d.cchild, 8 * cap) == hipSuccess;
-    ok = ok && dalloc(&d.sorted, (size_t)n) == hipSuccess && dalloc(&d.bnode, (size_t)n) == hipSuccess;
+    ok = ok && dalloc(&d.lvl_list, cap) == hipSuccess;
+    ok = ok && dalloc(&d.lvl_blockhist, (size_t)nb::kLevelBlocks * nb::kMaxDepth) == hipSuccess;
+    ok = ok && dalloc(&d.nodes4, (size_t)n + cap) == hipSuccess;
+    ok = ok && dalloc(&d.sorted, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.bbox_part, (size_t)6 * nb::kBBoxBlocks) == hipSuccess;
     if (!ok) {
         fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
         return bail(NB_E_HIP);
     }
-    // epoch 0 tags every slot and flag as stale; the first bounding box moves to epoch 1
+    // epoch 0 tags every slot as stale; the first bounding box moves to epoch 1
     ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(d.ready, 0, cap * sizeof(uint32_t), s->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(d.sready, 0, cap * sizeof(uint32_t), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int2), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.count, 0, cap * sizeof(int), s->stream) == hipSuccess;
     nb::Ctl c = initial_ctl(s, 0);
     ok = ok && hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream) == hipSuccess;
@@ -257,13 +291,6 @@ int nb_create(const nb_params *p, nb_sim **out)
 
     s->bbox_blocks = std::max(1, std::min(nb::kBBoxBlocks, (n + 2047) / 2048));
     s->build_blocks = build_threads(n) / 256;
-    // summarize and sort poll on each other's results inside one launch, so
-    // every block of the grid must be resident: stay at half the occupancy
-    // limit and at most 4 blocks per CU
-    const int sb = std::max(1, std::min(4, nb::summarize_blocks_per_cu(s->device) / 2));
-    const int tb = std::max(1, std::min(4, nb::sort_blocks_per_cu(s->device) / 2));
-    s->summarize_blocks = sb * s->cus;
-    s->sort_blocks = tb * s->cus;
     *out = s;
     return NB_OK;
 }
@@ -284,20 +311,28 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     if (!x || !y || !z || !vx || !vy || !vz || !m) return fail(s, NB_E_ARG, "null state array");
     HIP_TRY(s, hipSetDevice(s->device));
     nb::Dev &d = s->d;
-    const size_t bytes = (size_t)d.n * sizeof(float);
-    const float *src[] = {x, y, z, vx, vy, vz, m};
-    float *dst[] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m};
-    for (int i = 0; i < 7; ++i)
-        HIP_TRY(s, hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyHostToDevice, s->stream));
-    for (float *a : {d.ax, d.ay, d.az}) HIP_TRY(s, hipMemsetAsync(a, 0, bytes, s->stream));
-    if (d.fp64) {
-        std::vector<double> tmp((size_t)d.n);
-        const float *src3[] = {x, y, z};
-        double *dst3[] = {d.x64, d.y64, d.z64};
-        for (int k = 0; k < 3; ++k) {
-            for (int i = 0; i < d.n; ++i) tmp[i] = (double)src3[k][i];
-            HIP_TRY(s, hipMemcpy(dst3[k], tmp.data(), (size_t)d.n * sizeof(double), hipMemcpyHostToDevice));
+    const size_t n = (size_t)d.n;
+    {
+        std::vector<float4> pm(n), vel(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (size_t i = 0; i < n; ++i) {
+            pm[i] = make_float4(x[i], y[i], z[i], m[i]);
+            vel[i] = make_float4(vx[i], vy[i], vz[i], 0.f);
         }
+        HIP_TRY(s, hipMemcpy(d.pm4, pm.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(s, hipMemcpy(d.vel4, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(s, hipMemcpy(d.acc4, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    if (d.fp64) {
+        std::vector<double4> p64(n);
+        for (size_t i = 0; i < n; ++i) p64[i] = make_double4(x[i], y[i], z[i], 0.0);
+        HIP_TRY(s, hipMemcpy(d.pos64, p64.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    }
+    // the build inserts bodies in the previous step's sorted order; before the
+    // first step that order is the identity
+    {
+        std::vector<int> ident((size_t)d.n);
+        for (int i = 0; i < d.n; ++i) ident[i] = i;
+        HIP_TRY(s, hipMemcpy(d.sorted, ident.data(), (size_t)d.n * sizeof(int), hipMemcpyHostToDevice));
     }
     // keep the epoch running so that no slot of an earlier tree can look current
     nb::Ctl cur;
@@ -374,27 +409,23 @@ int nb_copy_vertices(nb_sim *s, float *pos4, float *vel4)
 
 int32_t nb_num_bodies(const nb_sim *s) { return s ? s->d.n : 0; }
 
-static int copy_back(nb_sim *s, void *dst, const void *src, size_t bytes)
-{
-    if (!dst) return NB_OK;
-    HIP_TRY(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
-    return NB_OK;
-}
-
 int nb_get_state(nb_sim *s, float *x, float *y, float *z, float *vx, float *vy, float *vz)
 {
     if (!s) return NB_E_ARG;
     HIP_TRY(s, hipSetDevice(s->device));
-    const size_t b = (size_t)s->d.n * sizeof(float);
-    int r = 0;
-    r |= copy_back(s, x, s->d.x, b);
-    r |= copy_back(s, y, s->d.y, b);
-    r |= copy_back(s, z, s->d.z, b);
-    r |= copy_back(s, vx, s->d.vx, b);
-    r |= copy_back(s, vy, s->d.vy, b);
-    r |= copy_back(s, vz, s->d.vz, b);
-    if (r) return NB_E_HIP;
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)s->d.n;
+    std::vector<float4> p, v;
+    int r = fetch(s, p, s->d.pm4, n);
+    if (!r) r = fetch(s, v, s->d.vel4, n);
+    if (r) return r;
+    for (size_t i = 0; i < n; ++i) {
+        if (x) x[i] = p[i].x;
+        if (y) y[i] = p[i].y;
+        if (z) z[i] = p[i].z;
+        if (vx) vx[i] = v[i].x;
+        if (vy) vy[i] = v[i].y;
+        if (vz) vz[i] = v[i].z;
+    }
     return NB_OK;
 }
 
@@ -402,10 +433,15 @@ int nb_get_accel(nb_sim *s, float *ax, float *ay, float *az)
 {
     if (!s) return NB_E_ARG;
     HIP_TRY(s, hipSetDevice(s->device));
-    const size_t b = (size_t)s->d.n * sizeof(float);
-    int r = copy_back(s, ax, s->d.ax, b) | copy_back(s, ay, s->d.ay, b) | copy_back(s, az, s->d.az, b);
-    if (r) return NB_E_HIP;
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)s->d.n;
+    std::vector<float4> a;
+    int r = fetch(s, a, s->d.acc4, n);
+    if (r) return r;
+    for (size_t i = 0; i < n; ++i) {
+        if (ax) ax[i] = a[i].x;
+        if (ay) ay[i] = a[i].y;
+        if (az) az[i] = a[i].z;
+    }
     return NB_OK;
 }
 
@@ -414,10 +450,15 @@ int nb_get_positions64(nb_sim *s, double *x, double *y, double *z)
     if (!s) return NB_E_ARG;
     if (!s->d.fp64) return fail(s, NB_E_STATE, "NB_FP64_POSITIONS is not set");
     HIP_TRY(s, hipSetDevice(s->device));
-    const size_t b = (size_t)s->d.n * sizeof(double);
-    int r = copy_back(s, x, s->d.x64, b) | copy_back(s, y, s->d.y64, b) | copy_back(s, z, s->d.z64, b);
-    if (r) return NB_E_HIP;
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    const size_t n = (size_t)s->d.n;
+    std::vector<double4> p;
+    int r = fetch(s, p, s->d.pos64, n);
+    if (r) return r;
+    for (size_t i = 0; i < n; ++i) {
+        if (x) x[i] = p[i].x;
+        if (y) y[i] = p[i].y;
+        if (z) z[i] = p[i].z;
+    }
     return NB_OK;
 }
 
@@ -473,18 +514,22 @@ int nb_get_cells(nb_sim *s, int32_t cap, int32_t *ncells, int32_t *count, float 
     nb::Ctl c;
     int r = read_ctl(s, &c);
     if (r) return r;
-    const int C = std::min(c.n_cells, s->d.cap);
-    *ncells = C;
-    if (cap < C) return fail(s, NB_E_ARG, "cell buffers hold %d of %d cells", cap, C);
-    std::vector<float4> node((size_t)C);
-    if (count) HIP_TRY(s, hipMemcpyAsync(count, s->d.count, (size_t)C * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(s, hipMemcpyAsync(node.data(), s->d.cnode, (size_t)C * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(s, hipStreamSynchronize(s->stream));
-    for (int i = 0; i < C; ++i) {
-        if (x) x[i] = node[i].x;
-        if (y) y[i] = node[i].y;
-        if (z) z[i] = node[i].z;
-        if (m) m[i] = node[i].w;
+    const int live = c.live_cells;
+    *ncells = live;
+    if (cap < live) return fail(s, NB_E_ARG, "cell buffers hold %d of %d cells", cap, live);
+    std::vector<int> list, cnt;
+    std::vector<float4> node;
+    r = fetch(s, list, s->d.lvl_list, (size_t)live);
+    if (!r) r = fetch(s, cnt, s->d.count, (size_t)s->d.cap);
+    if (!r) r = fetch(s, node, s->d.nodes4 + s->d.n, (size_t)s->d.cap);
+    if (r) return r;
+    for (int i = 0; i < live; ++i) {
+        const int k = list[i];
+        if (count) count[i] = cnt[k];
+        if (x) x[i] = node[k].x;
+        if (y) y[i] = node[k].y;
+        if (z) z[i] = node[k].z;
+        if (m) m[i] = node[k].w;
     }
     return NB_OK;
 }

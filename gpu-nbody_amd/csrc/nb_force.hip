@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
 
     const int pos = blockIdx.x * 256 + tid;
     const bool valid = pos < n;
-    const float4 me = valid ? d.bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 me = valid ? d.nodes4[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float eps2 = d.eps2;
     const unsigned long long gmask =
         (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << ((lane / G) * G));
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
         for (int k = 0; k < 8; ++k) {
             const int v = ch[k];
             if (v < 0) break;
-            const float4 q = (v < n) ? d.bnode[v] : d.cnode[v - n];
+            const float4 q = d.nodes4[v]; // body in sorted order, or cell v - n
             const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             bool acc = v < n;
@@ -94,9 +94,8 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
                 pz = fmaf(dz, f, pz);
                 ++cnt;
             } else {
-                // children always have larger cell indices than their parent and
                 // the tree is at most kMaxDepth deep: anything else is corrupt
-                if (sp >= kStack || v - n <= c || v - n >= d.cap || dep + 1 >= kMaxDepth) {
+                if (sp >= kStack || v - n >= d.cap || dep + 1 >= kMaxDepth) {
                     overflow = true;
                     break;
                 }
@@ -120,36 +119,37 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
         const int b = d.sorted[pos];
         const float fx = sx - kx, fy = sy - ky, fz = sz - kz;
         const float dt = d.dt;
-        float vx = d.vx[b], vy = d.vy[b], vz = d.vz[b];
+        const float4 v0 = d.vel4[b];
+        float vx = v0.x, vy = v0.y, vz = v0.z;
         if (ctl->step > 0) { // calculateforce.cl:174-179
-            vx = vx + ((fx - d.ax[b]) * dt) * 0.5f;
-            vy = vy + ((fy - d.ay[b]) * dt) * 0.5f;
-            vz = vz + ((fz - d.az[b]) * dt) * 0.5f;
+            const float4 a0 = d.acc4[b];
+            vx = vx + ((fx - a0.x) * dt) * 0.5f;
+            vy = vy + ((fy - a0.y) * dt) * 0.5f;
+            vz = vz + ((fz - a0.z) * dt) * 0.5f;
         }
-        d.ax[b] = fx; // calculateforce.cl:183-185
-        d.ay[b] = fy;
-        d.az[b] = fz;
+        d.acc4[b] = make_float4(fx, fy, fz, 0.0f); // calculateforce.cl:183-185
         if (fused) { // integrate.cl:29-43 for this body
             const float dvx = (fx * dt) * 0.5f, dvy = (fy * dt) * 0.5f, dvz = (fz * dt) * 0.5f;
             const float vhx = vx + dvx, vhy = vy + dvy, vhz = vz + dvz;
+            float4 p = me; // this body's (x, y, z, m), read from the sorted copy
             if (d.fp64) {
-                const double x = fma((double)vhx, (double)dt, d.x64[b]);
-                const double y = fma((double)vhy, (double)dt, d.y64[b]);
-                const double z = fma((double)vhz, (double)dt, d.z64[b]);
-                d.x64[b] = x; d.y64[b] = y; d.z64[b] = z;
-                d.x[b] = (float)x; d.y[b] = (float)y; d.z[b] = (float)z;
+                const double4 q = d.pos64[b];
+                const double x = fma((double)vhx, (double)dt, q.x);
+                const double y = fma((double)vhy, (double)dt, q.y);
+                const double z = fma((double)vhz, (double)dt, q.z);
+                d.pos64[b] = make_double4(x, y, z, 0.0);
+                p.x = (float)x; p.y = (float)y; p.z = (float)z;
             } else {
-                d.x[b] = fmaf(vhx, dt, d.x[b]);
-                d.y[b] = fmaf(vhy, dt, d.y[b]);
-                d.z[b] = fmaf(vhz, dt, d.z[b]);
+                p.x = fmaf(vhx, dt, p.x);
+                p.y = fmaf(vhy, dt, p.y);
+                p.z = fmaf(vhz, dt, p.z);
             }
+            d.pm4[b] = p;
             vx = vhx + dvx;
             vy = vhy + dvy;
             vz = vhz + dvz;
         }
-        d.vx[b] = vx;
-        d.vy[b] = vy;
-        d.vz[b] = vz;
+        d.vel4[b] = make_float4(vx, vy, vz, 0.0f);
     } else {
         ninter = 0;
     }

@@ -1,25 +1,31 @@
 // nb_internal.h -- device layout and kernel launch interface of the engine.
 //
-// HBM layout (all arrays are separate allocations, SoA):
-//   bodies   x,y,z,vx,vy,vz,ax,ay,az,m : float[N]      (original body order)
-//            x64,y64,z64               : double[N]     (NB_FP64_POSITIONS only)
-//   octree   slot  : uint64[8*cap]   child slots, (epoch << 32) | child
-//            ready : uint32[cap]     epoch at which summarize finished the cell
-//            sready: uint32[cap]     epoch at which sort gave the cell its start
-//            count : int32[cap]      bodies in the cell's subtree
-//            start : int32[cap]      first sorted position of the subtree
-//            com64 : double4[cap]    FP64 centre of mass (x,y,z) and mass (w)
-//            cnode : float4[cap]     FP32 centre of mass and mass (traversal copy)
-//            cchild: int32[8*cap]    compacted children for traversal: cells as
-//                                    N + cell, bodies as their sorted position
-//   sorted   sorted: int32[N]        body index at each sorted position
-//            bnode : float4[N]       (x,y,z,m) of bodies in sorted order
+// HBM layout (separate allocations):
+//   bodies   pm4   : float4[N]   (x, y, z, m), original body order
+//            vel4  : float4[N]   (vx, vy, vz, 0)
+//            acc4  : float4[N]   (ax, ay, az, 0)
+//            pos64 : double4[N]  FP64 positions (NB_FP64_POSITIONS only)
+//   octree   slot  : uint64[8*cap]  child slots, (epoch << 32) | child
+//            cinfo : int2[cap]      (parent, level | octant << 8), written when
+//                                   the cell is published
+//            count : int32[cap]     bodies in the cell's subtree
+//            start : int32[cap]     first sorted position of the subtree
+//            com64 : double4[cap]   FP64 centre of mass (x, y, z) and mass (w)
+//            cchild: int32[8*cap]   children compacted to the front; cells as
+//                                   N + cell, bodies as their sorted position
+//   levels   lvl_list : int32[cap]  live cells grouped by level
+//   traversal nodes4 : float4[N+cap] (x, y, z, m): bodies in sorted order at
+//                                   [0, N), centres of mass of cells at N + cell
+//            sorted : int32[N]     body index at each sorted position
+//
+// Body records are 16-byte AoS: streaming kernels still read them with fully
+// coalesced 16-byte lanes, and the gathers of the tree stages touch one cache
+// line per body instead of four.
 //
 // Node numbering follows the reference (buildtree.cl:277, sort.cl:223): a
 // child value v < 0 is empty, v < N is a body, v >= N is cell v - N. Cell 0 is
-// the root. Cells are allocated upward from 1, so every cell's index is larger
-// than its parent's (the reference allocates downward from NN with the same
-// property, buildtree.cl:109).
+// the root. Cells are taken from the pool upward from 1 (the reference takes
+// them downward from NN, buildtree.cl:109); cell numbers carry no order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,16 +33,17 @@
 
 namespace nb {
 
-constexpr int kMaxDepth = 64;       // reference MAXDEPTH, calculateforce.cl:12
+constexpr int kMaxDepth = 64;        // reference MAXDEPTH, calculateforce.cl:12
 constexpr int kBuildLevelLimit = 64; // cells live on levels < 64, bodies at depth <= 64
-constexpr int kBBoxBlocks = 1024;   // partial-result slots of the bounding box
+constexpr int kBBoxBlocks = 1024;    // partial-result slots of the bounding box
+constexpr int kLevelBlocks = 512;    // blocks of the level histogram / scatter
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 enum DevError : int {
     kErrNone = 0,
     kErrPool = 1,  // cell pool exhausted
     kErrDepth = 2, // tree too deep
-    kErrSync = 3,  // a bounded wait ran out
+    kErrSync = 3,  // inconsistent tree detected
     kErrStack = 4, // traversal stack overflow
 };
 
@@ -44,7 +51,7 @@ enum DevError : int {
 // one-element buffers: _step, _blockCount, _radius, _maxDepth, _bottom, _error).
 struct Ctl {
     int step;            // reference _step, starts at -1 (java:167)
-    uint32_t epoch;      // tree generation: tags child slots and ready flags
+    uint32_t epoch;      // tree generation: tags child slots
     int n_cells;         // cell allocation counter (reference _bottom)
     int max_depth;       // running maximum (reference _maxDepth, java:170)
     int error;           // DevError
@@ -53,9 +60,10 @@ struct Ctl {
     float root[3];
     float dq[kMaxDepth]; // opening thresholds (calculateforce.cl:52-67)
     unsigned long long interactions; // cumulative body-node interactions
-    int spin_limit;      // bound on the polling of summarize/sort
+    int levels;          // number of non-empty cell levels of the last tree
     int live_cells;      // cells linked into the tree by the last build
     int pad[2];
+    int lvl_start[kMaxDepth + 2]; // first lvl_list entry of each level
 };
 
 struct Dev {
@@ -64,32 +72,29 @@ struct Dev {
     float theta2, eps2, dt;
     int fp64;
     Ctl *ctl;
-    float *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m;
-    double *x64, *y64, *z64;
+    float4 *pm4, *vel4, *acc4;
+    double4 *pos64;
     unsigned long long *slot;
-    uint32_t *ready, *sready;
+    int2 *cinfo;
     int *count, *start;
     double4 *com64;
-    float4 *cnode;
     int *cchild;
+    int *lvl_list;
+    int *lvl_blockhist; // kLevelBlocks * kMaxDepth
+    float4 *nodes4;
     int *sorted;
-    float4 *bnode;
     float *bbox_part; // 6 * kBBoxBlocks
 };
 
 // ---- launchers (nb_tree.hip, nb_force.hip) ---------------------------------
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
 hipError_t launch_build(const Dev &d, int blocks, hipStream_t st);
-hipError_t launch_summarize(const Dev &d, int blocks, hipStream_t st);
-hipError_t launch_sort(const Dev &d, int blocks, hipStream_t st);
+hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
+hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st);
+hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st);
 hipError_t launch_force(const Dev &d, int vote, int fused, hipStream_t st);
 hipError_t launch_integrate(const Dev &d, hipStream_t st);
 hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st);
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st);
-hipError_t launch_init_state(const Dev &d, hipStream_t st);
-
-// persistent-grid sizes (all blocks must be co-resident for summarize/sort)
-int summarize_blocks_per_cu(int device);
-int sort_blocks_per_cu(int device);
 
 } // namespace nb

@@ -1,11 +1,13 @@
 // nb_tree.hip -- bounding box, octree build, summarize, sort, integrate and
 // the vertex export, for gfx950.
 //
-// Cross-workgroup communication inside a kernel uses HIP atomics at agent
-// scope only: child slots are published with a release CAS and read with
-// acquire loads; ready flags of summarize and sort are release stores read
-// with acquire loads. MI355X keeps one L2 per XCD, so nothing here relies on
-// plain stores becoming visible to another workgroup before a kernel ends.
+// Only the build communicates between workgroups inside a kernel, through
+// agent-scope atomics on the child slots and the cell counter. Summarize and
+// sort run one launch per tree level, so every value they read was written by
+// an earlier launch: no fences, no waiting on other threads. (On MI355X each
+// XCD has its own L2; an agent-scope acquire/release costs an L2 invalidate /
+// write-back, which is why the per-cell hand-offs of the reference's
+// summarize and sort are replaced by level launches.)
 #include "nb_internal.h"
 
 namespace nb {
@@ -34,6 +36,21 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int code)
     int expected = 0;
     __hip_atomic_compare_exchange_strong(&ctl->error, &expected, code, __ATOMIC_RELAXED,
                                          __ATOMIC_RELAXED, AGENT);
+}
+
+// A cell is in the tree when the slot its publisher recorded still names it.
+__device__ __forceinline__ bool cell_live(const Dev &d, int c, uint32_t ep, int *level)
+{
+    if (c == 0) {
+        *level = 0;
+        return true;
+    }
+    const int2 info = d.cinfo[c];
+    const int p = info.x;
+    if (p < 0 || p >= d.cap) return false;
+    const int oct = (info.y >> 8) & 7;
+    *level = info.y & 255;
+    return dec_slot(d.slot[(size_t)p * 8 + oct], ep) == d.n + c && *level < kMaxDepth;
 }
 
 // ---------------------------------------------------------------------------
@@ -83,11 +100,12 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
     __shared__ int is_last;
     const int n = d.n;
     // seeded with body 0 as the reference does (boundingbox.cl:44-58)
-    float v[6] = {d.x[0], d.y[0], d.z[0], d.x[0], d.y[0], d.z[0]};
+    const float4 b0 = d.pm4[0];
+    float v[6] = {b0.x, b0.y, b0.z, b0.x, b0.y, b0.z};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float px = d.x[i], py = d.y[i], pz = d.z[i];
-        v[0] = fminf(v[0], px); v[1] = fminf(v[1], py); v[2] = fminf(v[2], pz);
-        v[3] = fmaxf(v[3], px); v[4] = fmaxf(v[4], py); v[5] = fmaxf(v[5], pz);
+        const float4 p = d.pm4[i];
+        v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
+        v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
     }
     block_minmax(v, part);
     if (threadIdx.x == 0) {
@@ -125,8 +143,7 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
             raw = 0.25f * raw;
         }
         c->n_cells = 1; // the root is cell 0 (reference: _bottom = NUMBER_OF_NODES, :180)
-        c->live_cells = 0;
-        c->epoch += 1;  // every child slot and ready flag of the last tree turns stale
+        c->epoch += 1;  // every child slot of the last tree turns stale
         c->step += 1;   // boundingbox.cl:195
         c->bbox_count = 0;
     }
@@ -139,68 +156,128 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
 // epoch reads as empty, so the previous step's tree needs no clearing. Slot
 // states only move forward within a step: empty -> body -> cell. A body goes
 // into an empty slot with one CAS. A slot that holds a body d is split by
-// taking a fresh cell c (index larger than the parent's), placing d in c's
-// slot for its octant, and publishing c with a release CAS of d -> N+c; the
-// inserting body then continues inside c. A lost race leaves c unpublished:
-// it is cleared and kept as the thread's spare for a later split. Cell
-// centres are never read from memory: each thread derives them from the
-// root along its own path with the reference's arithmetic (buildtree.cl:124-136).
+// taking a cell c from the pool, placing d in c's slot for its octant, and
+// publishing c with a release CAS of d -> N+c; the inserting body then
+// continues inside c. A lost race leaves c unpublished: it is cleared and kept
+// as the thread's spare for its next split. Cell centres are never read from
+// memory: each thread derives them from the root along its own path with the
+// reference's arithmetic (buildtree.cl:124-136).
+//
+// Every decision to write a slot is a CAS against the value the thread read,
+// and the CAS compares with the current value; a slot that holds a cell never
+// changes again. A stale read therefore costs at most a failed CAS, so the
+// descent reads slots with ordinary cached loads and keeps, per thread, the
+// cells of its previous path in LDS: the next body of the chunk (its octant-
+// order neighbour) skips the levels the two paths share.
+//
+// Insertion order: thread t inserts a contiguous chunk of the previous step's
+// sorted (octant-path) order, so consecutive insertions of a thread share most
+// of their path and concurrent threads work in different regions.
 // ---------------------------------------------------------------------------
+constexpr int kPathCache = 24; // levels of the previous path kept per thread
+
+__device__ __forceinline__ int path_oct(unsigned long long w0, unsigned long long w1,
+                                        unsigned long long w2, int level)
+{
+    const unsigned long long w = level < 21 ? w0 : (level < 42 ? w1 : w2);
+    return (int)((w >> (3 * (level % 21))) & 7ull);
+}
+
+__device__ __forceinline__ void set_path_oct(unsigned long long &w0, unsigned long long &w1,
+                                             unsigned long long &w2, int level, int oct)
+{
+    const int sh = 3 * (level % 21);
+    const unsigned long long clr = ~(7ull << sh), val = (unsigned long long)oct << sh;
+    if (level < 21) w0 = (w0 & clr) | val;
+    else if (level < 42) w1 = (w1 & clr) | val;
+    else w2 = (w2 & clr) | val;
+}
+
+// Cells for the active lanes with one atomic per wave.
+__device__ __forceinline__ int alloc_cell(Ctl *ctl)
+{
+    const unsigned long long act = __ballot(1);
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)act) - 1;
+    const int rank = __popcll(act & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = __hip_atomic_fetch_add(&ctl->n_cells, __popcll(act), __ATOMIC_RELAXED, AGENT);
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
 __global__ __launch_bounds__(256) void k_build(Dev d)
 {
+    __shared__ int pc[kPathCache][256];
     const int n = d.n, cap = d.cap;
     Ctl *ctl = d.ctl;
     if (ctl->error) return;
     const uint32_t ep = ctl->epoch;
     const float radius = ctl->radius;
     const float rx = ctl->root[0], ry = ctl->root[1], rz = ctl->root[2];
+    const int tid = threadIdx.x;
+    const int T = gridDim.x * blockDim.x;
+    const int chunk = (n + T - 1) / T;
+    const int first = (blockIdx.x * blockDim.x + tid) * chunk;
+    const int last = min(n, first + chunk);
     int my_max = 0;
     int spare = -1;
     bool failed = false;
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < n && !failed;
-         b += gridDim.x * blockDim.x) {
-        const float bx = d.x[b], by = d.y[b], bz = d.z[b];
+    unsigned long long po0 = 0, po1 = 0, po2 = 0; // octants of the cached path
+    int plen = 0; // pc[1 .. plen-1] hold the cells of the cached path
+    for (int i = first; i < last && !failed; ++i) {
+        const int b = d.sorted[i];
+        const float4 me = d.pm4[b];
         int p = 0, level = 0;
         float cx = rx, cy = ry, cz = rz, r = radius;
+        // follow the previous path while the octants agree
+        while (level + 1 < plen) {
+            const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
+            if (oct != path_oct(po0, po1, po2, level)) break;
+            const float rh = r * 0.5f;
+            cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
+            cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
+            cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
+            r = rh;
+            p = pc[level + 1][tid];
+            ++level;
+        }
+        plen = level + 1;
         for (;;) {
-            const int oct = octant(cx, cy, cz, bx, by, bz);
+            const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
             unsigned long long *sp = d.slot + (size_t)p * 8 + oct;
-            unsigned long long w = __hip_atomic_load(sp, __ATOMIC_ACQUIRE, AGENT);
+            unsigned long long w = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             int v = dec_slot(w, ep);
             int next = -1; // cell to descend into
-            float ncx = 0.f, ncy = 0.f, ncz = 0.f;
             const float rh = r * 0.5f;
+            const float ncx = (cx - rh) + ((oct & 1) ? r : 0.0f); // buildtree.cl:124-136
+            const float ncy = (cy - rh) + ((oct & 2) ? r : 0.0f);
+            const float ncz = (cz - rh) + ((oct & 4) ? r : 0.0f);
             for (int tries = 0;; ++tries) {
                 if (tries > 16) { set_error(ctl, kErrSync); failed = true; break; }
                 if (v >= n) { next = v - n; break; }
                 if (v < 0) {
-                    if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, b), __ATOMIC_ACQUIRE,
-                                                             __ATOMIC_ACQUIRE, AGENT)) {
+                    if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, b), __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, AGENT)) {
                         my_max = max(my_max, level + 1);
                         break;
                     }
                     v = dec_slot(w, ep);
                     continue;
                 }
-                // slot holds body v: split it
+                // slot holds body v: split it (buildtree.cl:102-181)
                 if (level + 1 >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
-                int c;
-                if (spare > p) {
-                    c = spare;
-                    spare = -1;
-                } else {
-                    c = __hip_atomic_fetch_add(&ctl->n_cells, 1, __ATOMIC_RELAXED, AGENT);
-                    if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
-                    if (c <= p) { set_error(ctl, kErrSync); failed = true; break; }
-                }
-                ncx = (cx - rh) + ((oct & 1) ? r : 0.0f);
-                ncy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-                ncz = (cz - rh) + ((oct & 4) ? r : 0.0f);
-                const int od = octant(ncx, ncy, ncz, d.x[v], d.y[v], d.z[v]);
+                int c = spare;
+                if (c < 0) c = alloc_cell(ctl);
+                spare = -1;
+                if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
+                const float4 dv = d.pm4[v];
+                const int od = octant(ncx, ncy, ncz, dv.x, dv.y, dv.z);
                 unsigned long long *csl = d.slot + (size_t)c * 8 + od;
                 __hip_atomic_store(csl, tag(ep, v), __ATOMIC_RELAXED, AGENT);
-                if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_ACQ_REL,
-                                                         __ATOMIC_ACQUIRE, AGENT)) {
+                if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELEASE,
+                                                         __ATOMIC_RELAXED, AGENT)) {
+                    d.cinfo[c] = make_int2(p, (level + 1) | (oct << 8));
                     my_max = max(my_max, level + 2);
                     next = c;
                     break;
@@ -210,11 +287,14 @@ __global__ __launch_bounds__(256) void k_build(Dev d)
                 v = dec_slot(w, ep);
             }
             if (failed || next < 0) break;
-            if (next <= p || next >= cap) { set_error(ctl, kErrSync); failed = true; break; }
-            // descend (buildtree.cl:277-289): child centre from this cell's centre
-            cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
-            cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-            cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
+            if (next >= cap) { set_error(ctl, kErrSync); failed = true; break; }
+            // descend (buildtree.cl:277-289), remembering the path
+            set_path_oct(po0, po1, po2, level, oct);
+            if (level + 1 < kPathCache) {
+                pc[level + 1][tid] = next;
+                plen = level + 2;
+            }
+            cx = ncx; cy = ncy; cz = ncz;
             r = rh;
             p = next;
             ++level;
@@ -229,157 +309,168 @@ __global__ __launch_bounds__(256) void k_build(Dev d)
 }
 
 // ---------------------------------------------------------------------------
-// Stage 3: summarize (summarizetree.cl:15-179). Persistent grid; each thread
-// walks its cells from the highest index down. A cell is finished when every
-// child cell carries this step's ready flag; its centre of mass is then summed
-// in FP64 in child-slot order, stored, and its own flag is released. Children
-// always have larger indices than their parent, so the highest unfinished cell
-// can always proceed and the polling terminates (bounded by spin_limit).
+// Live cells grouped by level: per-block histograms, one scan, a scatter.
+// Blocks own contiguous ranges of cell numbers in both passes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_summarize(Dev d)
+__device__ __forceinline__ void block_range(int C, int *lo, int *hi)
 {
+    const int per = (C + kLevelBlocks - 1) / kLevelBlocks;
+    *lo = min(C, (int)blockIdx.x * per);
+    *hi = min(C, *lo + per);
+}
+
+__global__ __launch_bounds__(256) void k_level_hist(Dev d)
+{
+    __shared__ int h[kMaxDepth];
     Ctl *ctl = d.ctl;
-    if (ctl->error) return;
-    const int n = d.n;
+    if (threadIdx.x < kMaxDepth) h[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t ep = ctl->epoch;
     const int C = min(ctl->n_cells, d.cap);
-    const int T = gridDim.x * blockDim.x;
-    const int limit = ctl->spin_limit;
-    int c = C - 1 - (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    int spins = 0;
-    int live = 0;
-    while (c >= 0) {
-        const unsigned long long *sl = d.slot + (size_t)c * 8;
-        int ch[8];
-        bool ok = true;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) ch[k] = dec_slot(sl[k], ep);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (ch[k] >= n) {
-                const uint32_t r = __hip_atomic_load(d.ready + (ch[k] - n), __ATOMIC_RELAXED, AGENT);
-                ok = ok && (r == ep);
-            }
+    int lo, hi;
+    block_range(C, &lo, &hi);
+    if (!ctl->error) {
+        for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+            int lvl;
+            if (cell_live(d, c, ep, &lvl)) atomicAdd(&h[lvl], 1);
         }
-        if (!ok) {
-            if (++spins > limit) {
-                set_error(ctl, kErrSync);
-                return;
-            }
-            if ((spins & 1023) == 0 && __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, AGENT)) return;
-            continue;
-        }
-        // one acquire for all child flags read above (summarizetree.cl:86-102)
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        spins = 0;
-        double m = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        int cnt = 0, used = 0;
-        int *out = d.cchild + (size_t)c * 8;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = ch[k];
-            if (v < 0) continue;
-            out[used++] = v; // children compacted to the front (summarizetree.cl:75-81)
-            if (v < n) {
-                const double mb = (double)d.m[v];
-                m += mb;
-                sx = fma(mb, (double)d.x[v], sx);
-                sy = fma(mb, (double)d.y[v], sy);
-                sz = fma(mb, (double)d.z[v], sz);
-                cnt += 1;
-            } else {
-                const double4 q = d.com64[v - n];
-                m += q.w;
-                sx = fma(q.w, q.x, sx);
-                sy = fma(q.w, q.y, sy);
-                sz = fma(q.w, q.z, sz);
-                cnt += d.count[v - n];
-            }
-        }
-        for (int k = used; k < 8; ++k) out[k] = -1;
-        // a cell of zero mass gets its centre at the origin instead of the
-        // reference's 0 * inf = NaN (summarizetree.cl:161-167)
-        const double inv = m > 0.0 ? 1.0 / m : 0.0;
-        const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
-        d.com64[c] = com;
-        d.cnode[c] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)m);
-        d.count[c] = cnt;
-        __hip_atomic_store(d.ready + c, ep, __ATOMIC_RELEASE, AGENT); // summarizetree.cl:170-172
-        live += cnt > 0 ? 1 : 0;
-        c -= T;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) live += __shfl_xor(live, o);
-    if ((threadIdx.x & 63) == 0 && live) __hip_atomic_fetch_add(&ctl->live_cells, live, __ATOMIC_RELAXED, AGENT);
+    __syncthreads();
+    if (threadIdx.x < kMaxDepth) d.lvl_blockhist[blockIdx.x * kMaxDepth + threadIdx.x] = h[threadIdx.x];
+}
+
+// one block of kMaxDepth threads: thread L turns the column of level L into
+// global list offsets and totals
+__global__ __launch_bounds__(kMaxDepth) void k_level_scan(Dev d)
+{
+    __shared__ int tot[kMaxDepth];
+    const int L = threadIdx.x;
+    int s = 0;
+    for (int b = 0; b < kLevelBlocks; ++b) {
+        const int v = d.lvl_blockhist[b * kMaxDepth + L];
+        d.lvl_blockhist[b * kMaxDepth + L] = s;
+        s += v;
+    }
+    tot[L] = s;
+    __syncthreads();
+    if (L == 0) {
+        Ctl *ctl = d.ctl;
+        int acc = 0, levels = 0;
+        for (int k = 0; k < kMaxDepth; ++k) {
+            ctl->lvl_start[k] = acc;
+            acc += tot[k];
+            if (tot[k] > 0) levels = k + 1;
+        }
+        ctl->lvl_start[kMaxDepth] = acc;
+        ctl->levels = levels;
+        ctl->live_cells = acc;
+    }
+    __syncthreads();
+    const int base = d.ctl->lvl_start[L];
+    for (int b = 0; b < kLevelBlocks; ++b) d.lvl_blockhist[b * kMaxDepth + L] += base;
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(Dev d)
+{
+    __shared__ int cur[kMaxDepth];
+    Ctl *ctl = d.ctl;
+    if (threadIdx.x < kMaxDepth) cur[threadIdx.x] = d.lvl_blockhist[blockIdx.x * kMaxDepth + threadIdx.x];
+    __syncthreads();
+    if (ctl->error) return;
+    const uint32_t ep = ctl->epoch;
+    const int C = min(ctl->n_cells, d.cap);
+    int lo, hi;
+    block_range(C, &lo, &hi);
+    for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+        int lvl;
+        if (cell_live(d, c, ep, &lvl)) d.lvl_list[atomicAdd(&cur[lvl], 1)] = c;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Stage 4: sort (sort.cl:12-76). Persistent grid; each thread walks its cells
-// from the lowest index up. A cell whose start offset has arrived hands
-// consecutive offsets to its children in octant order: child cells get a
-// start and a ready flag, bodies get their sorted position, their (x,y,z,m)
-// in sorted order, and the traversal child list is rewritten to reference that
-// position. The result is the octant-path (Morton) order of the bodies.
+// Stage 3: summarize (summarizetree.cl:15-179), one launch per level from the
+// deepest up. A cell's centre of mass is summed in FP64 in child-slot order
+// from its children's results of the previous launch; children are compacted
+// to the front (summarizetree.cl:75-81) into the traversal list.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sort(Dev d)
+__global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int cells)
 {
-    Ctl *ctl = d.ctl;
-    if (ctl->error) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
     const int n = d.n;
-    const uint32_t ep = ctl->epoch;
-    const int C = min(ctl->n_cells, d.cap);
-    const int T = gridDim.x * blockDim.x;
-    const int limit = ctl->spin_limit;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    int spins = 0;
-    while (c < C) {
-        if (d.count[c] == 0) { // allocated but never linked into the tree
-            c += T;
-            continue;
-        }
-        int st;
-        if (c == 0) {
-            st = 0; // root, boundingbox.cl:189
+    const uint32_t ep = d.ctl->epoch;
+    const int c = d.lvl_list[d.ctl->lvl_start[level] + i];
+    const unsigned long long *sl = d.slot + (size_t)c * 8;
+    int ch[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ch[k] = dec_slot(sl[k], ep);
+    double m = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    int cnt = 0, used = 0;
+    int *out = d.cchild + (size_t)c * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v = ch[k];
+        if (v < 0) continue;
+        out[used++] = v;
+        if (v < n) {
+            const float4 b = d.pm4[v];
+            const double mb = (double)b.w;
+            m += mb;
+            sx = fma(mb, (double)b.x, sx);
+            sy = fma(mb, (double)b.y, sy);
+            sz = fma(mb, (double)b.z, sz);
+            cnt += 1;
         } else {
-            const uint32_t r = __hip_atomic_load(d.sready + c, __ATOMIC_RELAXED, AGENT); // sort.cl:215
-            if (r != ep) {
-                if (++spins > limit) {
-                    set_error(ctl, kErrSync);
-                    return;
-                }
-                if ((spins & 1023) == 0 && __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, AGENT)) return;
-                continue;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            st = d.start[c];
+            const double4 q = d.com64[v - n];
+            m += q.w;
+            sx = fma(q.w, q.x, sx);
+            sy = fma(q.w, q.y, sy);
+            sz = fma(q.w, q.z, sz);
+            cnt += d.count[v - n];
         }
-        spins = 0;
-        int *cc = d.cchild + (size_t)c * 8;
-        int ch[8];
+    }
+    for (int k = used; k < 8; ++k) out[k] = -1;
+    // a cell of zero mass gets its centre at the origin instead of the
+    // reference's 0 * inf = NaN (summarizetree.cl:161-167)
+    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+    const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
+    d.com64[c] = com;
+    d.nodes4[n + c] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)m);
+    d.count[c] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4: sort (sort.cl:12-76), one launch per level from the root down. A
+// cell's start arrived from its parent in the previous launch; it hands
+// consecutive offsets to its children in octant order: child cells get their
+// start, bodies their sorted position, their (x, y, z, m) in sorted order, and
+// the traversal list is rewritten to reference that position. The result is
+// the octant-path (Morton) order of the bodies.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const int n = d.n;
+    Ctl *ctl = d.ctl;
+    const int c = d.lvl_list[ctl->lvl_start[level] + i];
+    int st = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
+    if (level == 0) d.start[c] = 0;
+    int *cc = d.cchild + (size_t)c * 8;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ch[k] = cc[k];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = ch[k];
-            if (v < 0) break;
-            if (v >= n) { // sort.cl:223-237
-                d.start[v - n] = st;
-                st += d.count[v - n];
-            } else { // sort.cl:238-245
-                d.sorted[st] = v;
-                d.bnode[st] = make_float4(d.x[v], d.y[v], d.z[v], d.m[v]);
-                cc[k] = st;
-                ++st;
-            }
+    for (int k = 0; k < 8; ++k) {
+        const int v = cc[k];
+        if (v < 0) break;
+        if (v >= n) { // sort.cl:223-237
+            d.start[v - n] = st;
+            st += d.count[v - n];
+        } else { // sort.cl:238-245
+            if (st < 0 || st >= n) { set_error(ctl, kErrSync); return; }
+            d.sorted[st] = v;
+            d.nodes4[st] = d.pm4[v];
+            cc[k] = st;
+            ++st;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = ch[k];
-            if (v < 0) break;
-            if (v >= n) __hip_atomic_store(d.sready + (v - n), ep, __ATOMIC_RELAXED, AGENT);
-        }
-        c += T;
     }
 }
 
@@ -392,22 +483,24 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n) return;
     const float dt = d.dt;
-    const float dvx = (d.ax[i] * dt) * 0.5f, dvy = (d.ay[i] * dt) * 0.5f, dvz = (d.az[i] * dt) * 0.5f;
-    const float vhx = d.vx[i] + dvx, vhy = d.vy[i] + dvy, vhz = d.vz[i] + dvz;
+    const float4 a = d.acc4[i], v = d.vel4[i];
+    float4 p = d.pm4[i];
+    const float dvx = (a.x * dt) * 0.5f, dvy = (a.y * dt) * 0.5f, dvz = (a.z * dt) * 0.5f;
+    const float vhx = v.x + dvx, vhy = v.y + dvy, vhz = v.z + dvz;
     if (d.fp64) {
-        const double x = fma((double)vhx, (double)dt, d.x64[i]);
-        const double y = fma((double)vhy, (double)dt, d.y64[i]);
-        const double z = fma((double)vhz, (double)dt, d.z64[i]);
-        d.x64[i] = x; d.y64[i] = y; d.z64[i] = z;
-        d.x[i] = (float)x; d.y[i] = (float)y; d.z[i] = (float)z;
+        const double4 q = d.pos64[i];
+        const double x = fma((double)vhx, (double)dt, q.x);
+        const double y = fma((double)vhy, (double)dt, q.y);
+        const double z = fma((double)vhz, (double)dt, q.z);
+        d.pos64[i] = make_double4(x, y, z, 0.0);
+        p.x = (float)x; p.y = (float)y; p.z = (float)z;
     } else {
-        d.x[i] = fmaf(vhx, dt, d.x[i]);
-        d.y[i] = fmaf(vhy, dt, d.y[i]);
-        d.z[i] = fmaf(vhz, dt, d.z[i]);
+        p.x = fmaf(vhx, dt, p.x);
+        p.y = fmaf(vhy, dt, p.y);
+        p.z = fmaf(vhz, dt, p.z);
     }
-    d.vx[i] = vhx + dvx;
-    d.vy[i] = vhy + dvy;
-    d.vz[i] = vhz + dvz;
+    d.pm4[i] = p;
+    d.vel4[i] = make_float4(vhx + dvx, vhy + dvy, vhz + dvz, 0.0f);
 }
 
 // copyVertices (copyvertices.cl:7-17)
@@ -415,8 +508,14 @@ __global__ __launch_bounds__(256) void k_copy_vertices(Dev d, float4 *pos, float
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n) return;
-    if (pos) pos[i] = make_float4(d.x[i], d.y[i], d.z[i], 1.0f);
-    if (vel) vel[i] = make_float4(d.vx[i], d.vy[i], d.vz[i], 1.0f);
+    if (pos) {
+        const float4 p = d.pm4[i];
+        pos[i] = make_float4(p.x, p.y, p.z, 1.0f);
+    }
+    if (vel) {
+        const float4 v = d.vel4[i];
+        vel[i] = make_float4(v.x, v.y, v.z, 1.0f);
+    }
 }
 
 // Leaf membership: descend the child slots along each body's own octants
@@ -428,16 +527,16 @@ __global__ __launch_bounds__(256) void k_paths(Dev d, unsigned long long *paths3
     if (b >= d.n) return;
     const int n = d.n;
     const uint32_t ep = d.ctl->epoch;
-    const float bx = d.x[b], by = d.y[b], bz = d.z[b];
+    const float4 me = d.pm4[b];
     float cx = d.ctl->root[0], cy = d.ctl->root[1], cz = d.ctl->root[2], r = d.ctl->radius;
     unsigned long long w[3] = {0, 0, 0};
     int p = 0, level = 0, result = -1;
     while (level < 63) {
-        const int oct = octant(cx, cy, cz, bx, by, bz);
+        const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
         w[level / 21] |= (unsigned long long)oct << (3 * (level % 21));
         const int v = dec_slot(d.slot[(size_t)p * 8 + oct], ep);
         if (v == b) { result = level + 1; break; }
-        if (v < n || v - n <= p || v - n >= d.cap) break;
+        if (v < n || v - n >= d.cap) break;
         const float rh = r * 0.5f;
         cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
         cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
@@ -463,14 +562,23 @@ hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
     hipLaunchKernelGGL(k_build, dim3(blocks), dim3(256), 0, st, d);
     return hipGetLastError();
 }
-hipError_t launch_summarize(const Dev &d, int blocks, hipStream_t st)
+hipError_t launch_levels(const Dev &d, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_summarize, dim3(blocks), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_level_hist, dim3(kLevelBlocks), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(kMaxDepth), 0, st, d);
+    hipLaunchKernelGGL(k_level_scatter, dim3(kLevelBlocks), dim3(256), 0, st, d);
     return hipGetLastError();
 }
-hipError_t launch_sort(const Dev &d, int blocks, hipStream_t st)
+hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sort, dim3(blocks), dim3(256), 0, st, d);
+    if (cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_summarize_level, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+    return hipGetLastError();
+}
+hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)
+{
+    if (cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_level, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
     return hipGetLastError();
 }
 hipError_t launch_integrate(const Dev &d, hipStream_t st)
@@ -488,14 +596,5 @@ hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hi
     hipLaunchKernelGGL(k_paths, dim3((d.n + 255) / 256), dim3(256), 0, st, d, paths3, depth);
     return hipGetLastError();
 }
-
-static int blocks_per_cu(const void *kernel)
-{
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess) return 1;
-    return nb;
-}
-int summarize_blocks_per_cu(int) { return blocks_per_cu((const void *)k_summarize); }
-int sort_blocks_per_cu(int) { return blocks_per_cu((const void *)k_sort); }
 
 } // namespace nb

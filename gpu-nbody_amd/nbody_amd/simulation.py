@@ -224,7 +224,7 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
 
     def cells(self):
         nc = C.c_int32(0)
-        cap = int(max(self.counters().cells, 1))
+        cap = int(max(self.counters().live_cells, 1))
         cnt = np.zeros(cap, np.int32)
         xs = [np.zeros(cap, np.float32) for _ in range(4)]
         _check(self._lib, self._h, self._lib.nb_get_cells(self._h, cap, C.byref(nc), ptr(cnt),

@@ -139,9 +139,10 @@ int nb_get_sorted(nb_sim *sim, int32_t *sorted);
  * cell numbering depends on timing, the octant path does not. */
 int nb_get_paths(nb_sim *sim, uint64_t *paths3, int32_t *depth);
 
-/* Per-cell summary after summarize, in cell order: body count, centre of mass
- * (FP32) and mass. Arrays of at least `cap` entries; *ncells receives the
- * number of cells written (cells that are not in the tree have count 0). */
+/* Per-cell summary after summarize, one entry per cell of the tree (root
+ * first, then level by level): body count, centre of mass (FP32) and mass.
+ * Arrays of at least `cap` entries (any may be NULL); *ncells receives the
+ * number of cells. */
 int nb_get_cells(nb_sim *sim, int32_t cap, int32_t *ncells, int32_t *count, float *x,
                  float *y, float *z, float *m);
 
