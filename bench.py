@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X Barnes-Hut engine.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode learned|plain]
+
+Workload: 4,194,304 bodies in 64 Gaussian clusters (seeded), softening 0.01,
+dt 1e-3, FP32 traversal with Kahan-compensated accumulation, FP64
+summarize and FP64 leapfrog positions, vote width 64. A step is one full
+pass of the pipeline (bounding box, build, summarize, sort, force,
+kick-drift-kick) over all bodies; inputs are resident in HBM before the
+timed region starts. The metric compares two force methods, each of which
+must reach a median relative force error <= 1e-3 against the FP64 direct sum
+on 4096 sampled bodies:
+  plain   Barnes-Hut at theta = 0.4
+  learned Barnes-Hut at coarse theta = 0.9 plus the learned correction
+`value` is the steps/s of the requested mode (learned when the library
+provides it); the other mode's steps/s and both errors are in `config`.
+
+Prints one JSON line. One GPU only: --gpus N > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-nbody_amd"))
+
+METRIC = "steps/s @4M bodies: learned coarse-θ vs plain θ=0.4 at equal 1e-3 force error"
+N_BODIES = 4_194_304
+THETA_PLAIN = 0.4
+THETA_COARSE = 0.9
+SOFTENING = 0.01
+DT = 1e-3
+VOTE = 64
+FLOP_PER_INTERACTION = 20  # GPU Gems 3 ch. 31 convention (Nyland, Harris, Prins 2007)
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector packed = matrix dense) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_sim(state, mode, model=None):
+    import numpy as np  # noqa: F401
+    from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
+    from nbody_amd.universe import ArrayUniverse
+
+    kw = dict(theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote_width=VOTE,
+              flags=_lib.NB_FP64_POSITIONS | _lib.NB_FUSED_UPDATE)
+    if mode == "learned":
+        kw.update(mode=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE)
+    sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
+    sim.init(None)
+    if mode == "learned":
+        sim.set_model(model)
+    return sim
+
+
+def timed_run(sim, steps, warmup):
+    """W untimed steps, then exactly K steps between two synchronizations
+    (nb_step returns after its stream has drained)."""
+    sim.steps(warmup)
+    t0 = time.perf_counter()
+    sim.steps(steps)
+    t1 = time.perf_counter()
+    return steps / (t1 - t0), (t1 - t0) * 1e3 / steps
+
+
+def stage_profile(sim, steps):
+    """Per-stage device time (HIP events on the engine's stream) over `steps`."""
+    sim.set_timing(True)
+    sim.steps(steps)
+    sim.set_timing(False)
+    c = sim.counters()
+    names = ("bbox", "build", "summarize", "sort", "force", "integrate")
+    return {k: round(c.stage_ms[i], 4) for i, k in enumerate(names)}, c.interactions / steps
+
+
+def cpu_baseline(state, threads):
+    """The oracle (plain-C restatement of the reference step, OpenMP force) on
+    the host cores: a full tree build of the workload plus the force pass on
+    one vote group in four, scaled to a whole step."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.build()
+    pyoracle.lib().orc_set_threads(threads)
+    n = len(state[0])
+    o = pyoracle.Oracle(n, theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote=VOTE, flags=pyoracle.FP64_POS)
+    o.set_state(*state)
+    t0 = time.perf_counter()
+    assert o.run_stages(1, 4) == 0
+    t_tree = time.perf_counter() - t0
+    groups = (n + VOTE - 1) // VOTE
+    sample = np.arange(0, groups, 4, dtype=np.int32)
+    t0 = time.perf_counter()
+    assert o.force_groups(sample) == 0
+    t_force = (time.perf_counter() - t0) * groups / len(sample)
+    t0 = time.perf_counter()
+    o.run_stages(6, 6)
+    t_int = time.perf_counter() - t0
+    o.close()
+    step_s = t_tree + t_force + t_int
+    return {"value": round(1.0 / step_s, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle/bh_oracle.c (reference step restated in C, OpenMP force), plain theta "
+                       f"{THETA_PLAIN}, vote {VOTE}, N={n}: full bbox+build+summarize+sort "
+                       f"({t_tree:.2f} s) and integrate ({t_int:.2f} s), force on 1 of every 4 vote groups "
+                       f"({len(sample)} groups) scaled x{groups / len(sample):.0f} to {t_force:.1f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_BODIES)
+    ap.add_argument("--mode", choices=["learned", "plain"], default=None)
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import nbody_amd
+    from nbody_amd import universe as U
+
+    lib = nbody_amd.load()
+    has_learned = hasattr(lib, "nb_set_model")
+    mode = args.mode or ("learned" if has_learned else "plain")
+    if mode == "learned" and not has_learned:
+        raise SystemExit("bench.py: libnbody.so has no learned mode")
+
+    log(f"generating {args.n} clustered bodies")
+    state = U.gaussian_clusters(args.n, seed=4)
+
+    results = {}
+    model = None
+    if has_learned:
+        from nbody_amd import learned
+        model = learned.load_default_model() if mode == "learned" else None
+    for m in (["plain", "learned"] if mode == "learned" else ["plain"]):
+        log(f"[{m}] timing {args.steps} steps after {args.warmup} warm-up steps")
+        sim = make_sim(state, m, model)
+        sps, ms = timed_run(sim, args.steps, args.warmup)
+        stages, inter = stage_profile(sim, max(3, min(args.steps, 10)))
+        err = sim.force_error(args.samples, seed=1000)
+        results[m] = dict(steps_per_s=sps, ms_per_step=ms, stages=stages, interactions=inter, err=err)
+        log(f"[{m}] {sps:.3f} steps/s, {ms:.2f} ms/step, median err {err['median']:.2e}, stages {stages}")
+        sim.close()
+
+    main_r = results[mode]
+    force_ms = main_r["stages"]["force"]
+    achieved = FLOP_PER_INTERACTION * main_r["interactions"] / (force_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "force_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": round(main_r["steps_per_s"], 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded mixture of 64 Gaussian clusters",
+        "config": {
+            "workload": f"clustered universe, N={args.n}, {mode} force method",
+            "n_bodies": args.n,
+            "mode": mode,
+            "theta_plain": THETA_PLAIN,
+            "theta_coarse": THETA_COARSE,
+            "softening": SOFTENING,
+            "dt": DT,
+            "vote_width": VOTE,
+            "precision": "FP32 traversal (Kahan), FP64 summarize, FP64 positions",
+            "median_rel_force_err": round(main_r["err"]["median"], 8),
+            "p90_rel_force_err": round(main_r["err"]["p90"], 8),
+            "stage_ms": main_r["stages"],
+            "interactions_per_step": int(main_r["interactions"]),
+            "interactions_per_s": round(main_r["interactions"] / (force_ms * 1e-3), 1),
+        },
+        "roofline": {
+            "kernel": "force traversal (k_force64)",
+            "bound": "mfma",
+            "engine": "FP32 VALU (no matrix instructions); peak is MI355X FP32 157.3 TFLOP/s",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic,
+            "flop_per_interaction": FLOP_PER_INTERACTION,
+            "launch_ms": force_ms,
+        },
+    }
+    if mode == "learned":
+        p = results["plain"]
+        line["config"].update({
+            "plain_steps_per_s": round(p["steps_per_s"], 4),
+            "speedup_vs_plain": round(main_r["steps_per_s"] / p["steps_per_s"], 3),
+            "plain_median_rel_force_err": round(p["err"]["median"], 8),
+        })
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(state, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

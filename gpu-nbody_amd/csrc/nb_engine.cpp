@@ -7,6 +7,7 @@
 // reads the device error word once at the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -26,12 +27,15 @@ struct nb_sim {
     int bbox_blocks = 1, build_blocks = 1;
     bool have_state = false;
     bool timing = false;
-    hipEvent_t ev[8] = {};
+    std::vector<hipEvent_t> tev; // per-step stage events (timing mode)
     double stage_ms[8] = {};
     unsigned long long inter_before = 0;
     long long last_interactions = 0;
     long long fallbacks = 0;
     double last_check_median = 0.0;
+    int *chk_idx = nullptr;             // sampled-error check buffers, allocated on first use
+    double4 *chk_part = nullptr, *chk_out = nullptr;
+    int chk_cap = 0;
     int levels = 0;                      // non-empty cell levels of the current tree
     int lvl_start[nb::kMaxDepth + 2] = {}; // host copy of the level list offsets
     std::string err;
@@ -185,10 +189,11 @@ void free_all(nb_sim *s)
     nb::Dev &d = s->d;
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
-                    d.lvl_blockhist,  d.nodes4, d.sorted,        d.bbox_part};
+                    d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
+                    s->chk_idx, s->chk_part, s->chk_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    for (auto &e : s->ev)
+    for (auto &e : s->tev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -271,7 +276,8 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = ok && dalloc(&d.cchild, 8 * cap) == hipSuccess;
     ok = ok && dalloc(&d.lvl_list, cap) == hipSuccess;
     ok = ok && dalloc(&d.lvl_blockhist, (size_t)nb::kLevelBlocks * nb::kMaxDepth) == hipSuccess;
-    ok = ok && dalloc(&d.nodes4, (size_t)n + cap) == hipSuccess;
+    ok = ok && dalloc(&d.cdata, 8 * cap) == hipSuccess;
+    ok = ok && dalloc(&d.bnode, (size_t)n) == hipSuccess && dalloc(&d.probe, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.sorted, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.bbox_part, (size_t)6 * nb::kBBoxBlocks) == hipSuccess;
     if (!ok) {
@@ -286,8 +292,6 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = ok && hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(s->stream) == hipSuccess;
     if (!ok) return bail(NB_E_HIP);
-    for (auto &ev : s->ev)
-        if (hipEventCreate(&ev) != hipSuccess) return bail(NB_E_HIP);
 
     s->bbox_blocks = std::max(1, std::min(nb::kBBoxBlocks, (n + 2047) / 2048));
     s->build_blocks = build_threads(n) / 256;
@@ -363,20 +367,33 @@ int nb_step(nb_sim *s, int32_t nsteps)
     if (nsteps < 0) return fail(s, NB_E_ARG, "negative step count");
     if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
     HIP_TRY(s, hipSetDevice(s->device));
+    // with timing on, every stage of every step is bracketed by events on the
+    // handle's stream; stage_ms reports the per-step average
+    const size_t nev = s->timing ? (size_t)7 * nsteps : 0;
+    while (s->tev.size() < nev) {
+        hipEvent_t e;
+        HIP_TRY(s, hipEventCreate(&e));
+        s->tev.push_back(e);
+    }
     for (int k = 0; k < nsteps; ++k) {
-        const bool t = s->timing && k == nsteps - 1;
+        hipEvent_t *ev = s->timing ? s->tev.data() + 7 * (size_t)k : nullptr;
         for (int st = NB_STAGE_BBOX; st <= NB_STAGE_INTEGRATE; ++st) {
-            if (t) HIP_TRY(s, hipEventRecord(s->ev[st - 1], s->stream));
+            if (ev) HIP_TRY(s, hipEventRecord(ev[st - 1], s->stream));
             int r = launch_stage(s, st);
             if (r) return r;
         }
-        if (t) HIP_TRY(s, hipEventRecord(s->ev[6], s->stream));
+        if (ev) HIP_TRY(s, hipEventRecord(ev[6], s->stream));
     }
     int r = check_device_error(s);
     if (s->timing && nsteps > 0) {
         for (int st = 0; st < 6; ++st) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, s->ev[st], s->ev[st + 1]) == hipSuccess) s->stage_ms[st] = ms;
+            double tot = 0.0;
+            for (int k = 0; k < nsteps; ++k) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, s->tev[7 * (size_t)k + st], s->tev[7 * (size_t)k + st + 1]) == hipSuccess)
+                    tot += ms;
+            }
+            s->stage_ms[st] = tot / nsteps;
         }
     }
     return r;
@@ -518,19 +535,148 @@ int nb_get_cells(nb_sim *s, int32_t cap, int32_t *ncells, int32_t *count, float 
     *ncells = live;
     if (cap < live) return fail(s, NB_E_ARG, "cell buffers hold %d of %d cells", cap, live);
     std::vector<int> list, cnt;
-    std::vector<float4> node;
+    std::vector<double4> com;
     r = fetch(s, list, s->d.lvl_list, (size_t)live);
     if (!r) r = fetch(s, cnt, s->d.count, (size_t)s->d.cap);
-    if (!r) r = fetch(s, node, s->d.nodes4 + s->d.n, (size_t)s->d.cap);
+    if (!r) r = fetch(s, com, s->d.com64, (size_t)s->d.cap);
     if (r) return r;
     for (int i = 0; i < live; ++i) {
         const int k = list[i];
         if (count) count[i] = cnt[k];
-        if (x) x[i] = node[k].x;
-        if (y) y[i] = node[k].y;
-        if (z) z[i] = node[k].z;
-        if (m) m[i] = node[k].w;
+        if (x) x[i] = (float)com[k].x;
+        if (y) y[i] = (float)com[k].y;
+        if (z) z[i] = (float)com[k].z;
+        if (m) m[i] = (float)com[k].w;
     }
+    return NB_OK;
+}
+
+// ---- force-error measurement ------------------------------------------------
+
+// Bounding box .. sort and a force traversal in probe mode on the current
+// positions: accelerations go to d.probe, bodies are not advanced, and the
+// step counter is put back so the simulation's own stepping is unaffected.
+static int probe_pass(nb_sim *s)
+{
+    nb::Ctl c0;
+    int r = read_ctl(s, &c0);
+    if (r) return r;
+    for (int st = NB_STAGE_BBOX; st <= NB_STAGE_SORT; ++st) {
+        r = launch_stage(s, st);
+        if (r) return r;
+    }
+    hipError_t e = nb::launch_force(s->d, s->p.vote_width, 2, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "probe force launch: %s", hipGetErrorString(e));
+    HIP_TRY(s, hipMemcpyAsync(&s->d.ctl->step, &c0.step, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    return check_device_error(s);
+}
+
+static int ensure_check_buffers(nb_sim *s, int ns)
+{
+    if (s->chk_cap >= ns) return NB_OK;
+    if (s->chk_idx) (void)hipFree(s->chk_idx);
+    if (s->chk_part) (void)hipFree(s->chk_part);
+    if (s->chk_out) (void)hipFree(s->chk_out);
+    s->chk_idx = nullptr;
+    s->chk_part = s->chk_out = nullptr;
+    s->chk_cap = 0;
+    HIP_TRY(s, hipMalloc((void **)&s->chk_idx, (size_t)ns * sizeof(int)));
+    HIP_TRY(s, hipMalloc((void **)&s->chk_part, (size_t)ns * 64 * sizeof(double4)));
+    HIP_TRY(s, hipMalloc((void **)&s->chk_out, (size_t)ns * sizeof(double4)));
+    s->chk_cap = ns;
+    return NB_OK;
+}
+
+static int direct_accel(nb_sim *s, const int32_t *idx, int32_t ns, std::vector<double4> &out)
+{
+    int r = ensure_check_buffers(s, ns);
+    if (r) return r;
+    HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx, (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "direct-sum launch: %s", hipGetErrorString(e));
+    return fetch(s, out, (const double4 *)s->chk_out, (size_t)ns);
+}
+
+int nb_probe_accel(nb_sim *s, float *ax, float *ay, float *az)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    int r = probe_pass(s);
+    if (r) return r;
+    std::vector<float4> a;
+    r = fetch(s, a, (const float4 *)s->d.probe, (size_t)s->d.n);
+    if (r) return r;
+    for (size_t i = 0; i < a.size(); ++i) {
+        if (ax) ax[i] = a[i].x;
+        if (ay) ay[i] = a[i].y;
+        if (az) az[i] = a[i].z;
+    }
+    return NB_OK;
+}
+
+int nb_direct_accel(nb_sim *s, const int32_t *idx, int32_t ns, double *out3)
+{
+    if (!s || !idx || !out3 || ns < 0) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    for (int i = 0; i < ns; ++i)
+        if (idx[i] < 0 || idx[i] >= s->d.n) return fail(s, NB_E_ARG, "sample index %d out of range", idx[i]);
+    if (ns == 0) return NB_OK;
+    HIP_TRY(s, hipSetDevice(s->device));
+    std::vector<double4> out;
+    int r = direct_accel(s, idx, ns, out);
+    if (r) return r;
+    for (int i = 0; i < ns; ++i) {
+        out3[3 * i + 0] = out[i].x;
+        out3[3 * i + 1] = out[i].y;
+        out3[3 * i + 2] = out[i].z;
+    }
+    return NB_OK;
+}
+
+int nb_force_error(nb_sim *s, int32_t nsamples, uint64_t seed, double *stats4)
+{
+    if (!s || !stats4 || nsamples < 1) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int n = s->d.n;
+    const int ns = std::min(nsamples, n);
+    // ns distinct bodies: partial Fisher-Yates shuffle driven by splitmix64
+    std::vector<int32_t> perm((size_t)n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    uint64_t x = seed;
+    for (int i = 0; i < ns; ++i) {
+        x += 0x9e3779b97f4a7c15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        const int j = i + (int)(z % (uint64_t)(n - i));
+        std::swap(perm[i], perm[j]);
+    }
+    int r = probe_pass(s);
+    if (r) return r;
+    std::vector<double4> ref;
+    r = direct_accel(s, perm.data(), ns, ref);
+    if (r) return r;
+    std::vector<float4> a;
+    r = fetch(s, a, (const float4 *)s->d.probe, (size_t)n);
+    if (r) return r;
+    std::vector<double> err((size_t)ns);
+    double mean = 0.0;
+    for (int i = 0; i < ns; ++i) {
+        const float4 g = a[perm[i]];
+        const double dx = g.x - ref[i].x, dy = g.y - ref[i].y, dz = g.z - ref[i].z;
+        const double den = std::sqrt(ref[i].x * ref[i].x + ref[i].y * ref[i].y + ref[i].z * ref[i].z);
+        err[i] = std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(den, 1e-300);
+        mean += err[i];
+    }
+    std::sort(err.begin(), err.end());
+    stats4[0] = err[(size_t)(ns - 1) / 2];            // median
+    stats4[1] = err[(size_t)((ns - 1) * 0.9)];        // 90th percentile
+    stats4[2] = err[(size_t)ns - 1];                  // maximum
+    stats4[3] = mean / ns;                            // mean
+    s->last_check_median = stats4[0];
     return NB_OK;
 }
 

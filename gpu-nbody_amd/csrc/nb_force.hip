@@ -1,4 +1,5 @@
-// nb_force.hip -- force traversal (calculateforce.cl:25-188) for gfx950.
+// nb_force.hip -- force traversal (calculateforce.cl:25-188) for gfx950, and
+// the FP64 direct sum used to measure force errors.
 //
 // A block of 256 threads covers 256 consecutive sorted positions. A vote group
 // is G consecutive lanes (G = 16, 32 or 64; 16 is the reference's work-group
@@ -8,6 +9,12 @@
 // accepted only if every lane of the group has d^2 + eps^2 >= dq[depth], and an
 // opened cell is pushed. Which nodes interact therefore matches the reference
 // exactly; only the order of the terms differs.
+//
+// A popped cell's children indices and their (x, y, z, m) are contiguous
+// (cchild, cdata: 32 + 128 bytes). With G = 64 the whole traversal state is
+// wave-uniform: the cell comes off the LDS stack into a scalar register and
+// its 160 bytes arrive through scalar loads, so the vector ALU only does the
+// per-body arithmetic and the opening vote is one ballot.
 //
 // Accumulation: the terms of one popped cell (<= 8) are summed in FP32 and the
 // batch sum is added into a Kahan-compensated FP32 accumulator.
@@ -27,8 +34,154 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float x)
     sum = t;
 }
 
+__device__ __forceinline__ void flag_error(Ctl *ctl, int code)
+{
+    int expected = 0;
+    __hip_atomic_compare_exchange_strong(&ctl->error, &expected, code, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, AGENT);
+}
+
+// Epilogue for one body: velocity correction and store (calculateforce.cl:
+// 174-185), optionally the kick-drift-kick of integrate.cl:29-43 (mode 1), or
+// only the acceleration into the probe buffer (mode 2).
+__device__ __forceinline__ void finish_body(const Dev &d, int pos, float4 me, float fx, float fy,
+                                            float fz, int mode, int step)
+{
+    const int b = d.sorted[pos];
+    if (mode == 2) {
+        d.probe[b] = make_float4(fx, fy, fz, 0.0f);
+        return;
+    }
+    const float dt = d.dt;
+    const float4 v0 = d.vel4[b];
+    float vx = v0.x, vy = v0.y, vz = v0.z;
+    if (step > 0) {
+        const float4 a0 = d.acc4[b];
+        vx = vx + ((fx - a0.x) * dt) * 0.5f;
+        vy = vy + ((fy - a0.y) * dt) * 0.5f;
+        vz = vz + ((fz - a0.z) * dt) * 0.5f;
+    }
+    d.acc4[b] = make_float4(fx, fy, fz, 0.0f);
+    if (mode == 1) {
+        const float dvx = (fx * dt) * 0.5f, dvy = (fy * dt) * 0.5f, dvz = (fz * dt) * 0.5f;
+        const float vhx = vx + dvx, vhy = vy + dvy, vhz = vz + dvz;
+        float4 p = me; // this body's (x, y, z, m), from the sorted copy
+        if (d.fp64) {
+            const double4 q = d.pos64[b];
+            const double x = fma((double)vhx, (double)dt, q.x);
+            const double y = fma((double)vhy, (double)dt, q.y);
+            const double z = fma((double)vhz, (double)dt, q.z);
+            d.pos64[b] = make_double4(x, y, z, 0.0);
+            p.x = (float)x; p.y = (float)y; p.z = (float)z;
+        } else {
+            p.x = fmaf(vhx, dt, p.x);
+            p.y = fmaf(vhy, dt, p.y);
+            p.z = fmaf(vhz, dt, p.z);
+        }
+        d.pm4[b] = p;
+        vx = vhx + dvx;
+        vy = vhy + dvy;
+        vz = vhz + dvz;
+    }
+    d.vel4[b] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Vote width 64: one wave, one group, uniform traversal.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ cchild,
+                                                 const float4 *__restrict__ cdata,
+                                                 const float4 *__restrict__ bnode, int mode)
+{
+    __shared__ unsigned stk[4][kStack];
+    __shared__ float dq_s[kMaxDepth];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int n = d.n;
+    Ctl *ctl = d.ctl;
+    if (ctl->error) return;
+    if (ctl->max_depth > kMaxDepth) { // calculateforce.cl:69-73
+        if (tid == 0) flag_error(ctl, kErrDepth);
+        return;
+    }
+    if (tid < kMaxDepth) dq_s[tid] = ctl->dq[tid];
+    const int step = ctl->step;
+    __syncthreads();
+
+    const int pos = blockIdx.x * 256 + tid;
+    const bool valid = pos < n;
+    const float4 me = valid ? bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float eps2 = d.eps2;
+    const int cap = d.cap;
+
+    if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
+    int sp = 1;
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    long long ninter = 0; // per wave: every lane meets the same nodes
+    bool bad = false;
+    while (sp > 0) {
+        __builtin_amdgcn_wave_barrier();
+        --sp;
+        const unsigned e = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
+        const int c = (int)(e >> 6);
+        const int dep = (int)(e & 63u);
+        const float thr = dq_s[dep];
+        const int *chp = cchild + (size_t)c * 8;
+        const float4 *cdp = cdata + (size_t)c * 8;
+        int ch[8];
+        float4 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ch[k] = __builtin_amdgcn_readfirstlane(chp[k]);
+            q[k] = cdp[k];
+        }
+        float px = 0.f, py = 0.f, pz = 0.f;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = ch[k];
+            if (v < 0) break;
+            const float dx = q[k].x - me.x, dy = q[k].y - me.y, dz = q[k].z - me.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            bool acc = v < n;
+            if (!acc) acc = __ballot(!valid || d2 >= thr) == __ballot(1);
+            if (acc) {
+                const float ri = __builtin_amdgcn_rsqf(d2);
+                const float f = q[k].w * (ri * ri * ri);
+                px = fmaf(dx, f, px);
+                py = fmaf(dy, f, py);
+                pz = fmaf(dz, f, pz);
+                ++cnt;
+            } else {
+                // the tree is at most kMaxDepth deep: anything else is corrupt
+                if (sp >= kStack || v - n >= cap || dep + 1 >= kMaxDepth) {
+                    bad = true;
+                    break;
+                }
+                if (lane == 0) stk[wv][sp] = ((unsigned)(v - n) << 6) | (unsigned)(dep + 1);
+                ++sp;
+            }
+        }
+        kahan_add(sx, kx, px);
+        kahan_add(sy, ky, py);
+        kahan_add(sz, kz, pz);
+        ninter += cnt;
+        if (bad) break;
+    }
+    if (bad && lane == 0) flag_error(ctl, kErrStack);
+    if (valid) finish_body(d, pos, me, sx - kx, sy - ky, sz - kz, mode, step);
+    // device interaction counter: one atomic per wave
+    const unsigned long long nvalid = (unsigned long long)__popcll(__ballot(valid));
+    if (lane == 0 && ninter && nvalid)
+        __hip_atomic_fetch_add(&ctl->interactions, (unsigned long long)ninter * nvalid, __ATOMIC_RELAXED, AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Vote width 16 or 32: several groups per wave, each with its own stack.
+// ---------------------------------------------------------------------------
 template <int G>
-__global__ __launch_bounds__(256) void k_force(Dev d, int fused)
+__global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
 {
     constexpr int kGroups = 256 / G;
     __shared__ unsigned stk[kGroups][kStack];
@@ -40,29 +193,25 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
     Ctl *ctl = d.ctl;
     if (ctl->error) return;
     if (ctl->max_depth > kMaxDepth) { // calculateforce.cl:69-73
-        if (tid == 0) {
-            int expected = 0;
-            __hip_atomic_compare_exchange_strong(&ctl->error, &expected, (int)kErrDepth,
-                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED, AGENT);
-        }
+        if (tid == 0) flag_error(ctl, kErrDepth);
         return;
     }
     if (tid < kMaxDepth) dq_s[tid] = ctl->dq[tid];
+    const int step = ctl->step;
     __syncthreads();
 
     const int pos = blockIdx.x * 256 + tid;
     const bool valid = pos < n;
-    const float4 me = valid ? d.nodes4[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 me = valid ? d.bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float eps2 = d.eps2;
-    const unsigned long long gmask =
-        (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << ((lane / G) * G));
+    const unsigned long long gmask = ((1ull << G) - 1ull) << ((lane / G) * G);
     const bool leader = (lane % G) == 0;
 
     if (leader) stk[gl][0] = 0; // root cell 0 at depth 0
     int sp = 1;
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int ninter = 0;
-    bool overflow = false;
+    bool bad = false;
     while (sp > 0) {
         __builtin_amdgcn_wave_barrier();
         const unsigned e = stk[gl][--sp];
@@ -72,20 +221,18 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
         const int4 c0 = *reinterpret_cast<const int4 *>(d.cchild + (size_t)c * 8);
         const int4 c1 = *reinterpret_cast<const int4 *>(d.cchild + (size_t)c * 8 + 4);
         const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        const float4 *cdp = d.cdata + (size_t)c * 8;
         float px = 0.f, py = 0.f, pz = 0.f;
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int v = ch[k];
             if (v < 0) break;
-            const float4 q = d.nodes4[v]; // body in sorted order, or cell v - n
+            const float4 q = cdp[k];
             const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             bool acc = v < n;
-            if (!acc) {
-                const bool pred = !valid || d2 >= thr;
-                acc = (__ballot(pred) & gmask) == gmask;
-            }
+            if (!acc) acc = (__ballot(!valid || d2 >= thr) & gmask) == gmask;
             if (acc) {
                 const float ri = __builtin_amdgcn_rsqf(d2);
                 const float f = q.w * (ri * ri * ri);
@@ -94,9 +241,8 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
                 pz = fmaf(dz, f, pz);
                 ++cnt;
             } else {
-                // the tree is at most kMaxDepth deep: anything else is corrupt
                 if (sp >= kStack || v - n >= d.cap || dep + 1 >= kMaxDepth) {
-                    overflow = true;
+                    bad = true;
                     break;
                 }
                 if (leader) stk[gl][sp] = ((unsigned)(v - n) << 6) | (unsigned)(dep + 1);
@@ -107,68 +253,113 @@ __global__ __launch_bounds__(256) void k_force(Dev d, int fused)
         kahan_add(sy, ky, py);
         kahan_add(sz, kz, pz);
         ninter += cnt;
-        if (overflow) break;
+        if (bad) break;
     }
-    if (overflow && leader) {
-        int expected = 0;
-        __hip_atomic_compare_exchange_strong(&ctl->error, &expected, (int)kErrStack, __ATOMIC_RELAXED,
-                                             __ATOMIC_RELAXED, AGENT);
-    }
-
-    if (valid) {
-        const int b = d.sorted[pos];
-        const float fx = sx - kx, fy = sy - ky, fz = sz - kz;
-        const float dt = d.dt;
-        const float4 v0 = d.vel4[b];
-        float vx = v0.x, vy = v0.y, vz = v0.z;
-        if (ctl->step > 0) { // calculateforce.cl:174-179
-            const float4 a0 = d.acc4[b];
-            vx = vx + ((fx - a0.x) * dt) * 0.5f;
-            vy = vy + ((fy - a0.y) * dt) * 0.5f;
-            vz = vz + ((fz - a0.z) * dt) * 0.5f;
-        }
-        d.acc4[b] = make_float4(fx, fy, fz, 0.0f); // calculateforce.cl:183-185
-        if (fused) { // integrate.cl:29-43 for this body
-            const float dvx = (fx * dt) * 0.5f, dvy = (fy * dt) * 0.5f, dvz = (fz * dt) * 0.5f;
-            const float vhx = vx + dvx, vhy = vy + dvy, vhz = vz + dvz;
-            float4 p = me; // this body's (x, y, z, m), read from the sorted copy
-            if (d.fp64) {
-                const double4 q = d.pos64[b];
-                const double x = fma((double)vhx, (double)dt, q.x);
-                const double y = fma((double)vhy, (double)dt, q.y);
-                const double z = fma((double)vhz, (double)dt, q.z);
-                d.pos64[b] = make_double4(x, y, z, 0.0);
-                p.x = (float)x; p.y = (float)y; p.z = (float)z;
-            } else {
-                p.x = fmaf(vhx, dt, p.x);
-                p.y = fmaf(vhy, dt, p.y);
-                p.z = fmaf(vhz, dt, p.z);
-            }
-            d.pm4[b] = p;
-            vx = vhx + dvx;
-            vy = vhy + dvy;
-            vz = vhz + dvz;
-        }
-        d.vel4[b] = make_float4(vx, vy, vz, 0.0f);
-    } else {
-        ninter = 0;
-    }
-    // device interaction counter: one atomic per wave
-    unsigned long long tot = (unsigned long long)ninter;
+    if (bad && leader) flag_error(ctl, kErrStack);
+    if (valid) finish_body(d, pos, me, sx - kx, sy - ky, sz - kz, mode, step);
+    unsigned long long tot = valid ? (unsigned long long)ninter : 0ull;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
     if (lane == 0 && tot) __hip_atomic_fetch_add(&ctl->interactions, tot, __ATOMIC_RELAXED, AGENT);
 }
 
-hipError_t launch_force(const Dev &d, int vote, int fused, hipStream_t st)
+hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
 {
     const dim3 grid((d.n + 255) / 256), block(256);
     switch (vote) {
-    case 16: hipLaunchKernelGGL(k_force<16>, grid, block, 0, st, d, fused); break;
-    case 32: hipLaunchKernelGGL(k_force<32>, grid, block, 0, st, d, fused); break;
-    case 64: hipLaunchKernelGGL(k_force<64>, grid, block, 0, st, d, fused); break;
+    case 16: hipLaunchKernelGGL(k_force_g<16>, grid, block, 0, st, d, mode); break;
+    case 32: hipLaunchKernelGGL(k_force_g<32>, grid, block, 0, st, d, mode); break;
+    case 64:
+        hipLaunchKernelGGL(k_force64, grid, block, 0, st, d, (const int *)d.cchild, (const float4 *)d.cdata,
+                           (const float4 *)d.bnode, mode);
+        break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum on sampled bodies (the error yardstick; G = 1).
+// Block (t, s) sums the bodies [s * chunk, (s + 1) * chunk) for 64 sampled
+// targets: 4 threads per target stride through the chunk, whose bodies are
+// staged through LDS once for all 64 targets. Partial sums per (target,
+// chunk) are combined by a second kernel in a fixed order, so the result is
+// deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kDirChunks = 64;  // source chunks
+constexpr int kDirTargets = 64; // targets per block
+constexpr int kDirTile = 256;   // bodies staged per LDS tile
+
+__global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__ idx, int ns, double4 *part)
+{
+    __shared__ float4 tile[kDirTile];
+    __shared__ double4 red[256];
+    const int n = d.n;
+    const int t0 = blockIdx.x * kDirTargets;
+    const int chunk = (n + kDirChunks - 1) / kDirChunks;
+    const int lo = min(n, (int)blockIdx.y * chunk), hi = min(n, lo + chunk);
+    const int tl = threadIdx.x & (kDirTargets - 1); // target of this thread
+    const int sub = threadIdx.x / kDirTargets;      // 0..3: which quarter of each tile
+    const int tgt = t0 + tl;
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    if (tgt < ns) {
+        const int b = idx[tgt];
+        if (b >= 0 && b < n) {
+            const float4 p = d.pm4[b];
+            tx = p.x; ty = p.y; tz = p.z;
+        }
+    }
+    const double eps2 = (double)d.eps2;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int base = lo; base < hi; base += kDirTile) {
+        __syncthreads();
+        const int j = base + threadIdx.x;
+        tile[threadIdx.x] = j < hi ? d.pm4[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const int cnt = min(kDirTile, hi - base);
+        for (int k = sub; k < cnt; k += 4) {
+            const float4 s = tile[k];
+            const double dx = (double)s.x - tx, dy = (double)s.y - ty, dz = (double)s.z - tz;
+            const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+            const double ri = rsqrt(r2);
+            const double f = (double)s.w * ri * ri * ri;
+            ax = fma(dx, f, ax);
+            ay = fma(dy, f, ay);
+            az = fma(dz, f, az);
+        }
+    }
+    red[threadIdx.x] = make_double4(ax, ay, az, 0.0);
+    __syncthreads();
+    if (sub == 0 && tgt < ns) {
+        double4 r = red[tl];
+        for (int k = 1; k < 4; ++k) {
+            const double4 o = red[tl + k * kDirTargets];
+            r.x += o.x; r.y += o.y; r.z += o.z;
+        }
+        part[(size_t)tgt * kDirChunks + blockIdx.y] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_direct64_reduce(const double4 *part, int ns, double4 *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns) return;
+    double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = 0; k < kDirChunks; ++k) {
+        const double4 o = part[(size_t)t * kDirChunks + k];
+        r.x += o.x; r.y += o.y; r.z += o.z;
+    }
+    out[t] = r;
+}
+
+// part must hold ns * 64 entries, out ns entries
+hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, hipStream_t st)
+{
+    if (ns <= 0) return hipSuccess;
+    const dim3 grid((ns + kDirTargets - 1) / kDirTargets, kDirChunks);
+    hipLaunchKernelGGL(k_direct64, grid, dim3(256), 0, st, d, idx, ns, part);
+    hipLaunchKernelGGL(k_direct64_reduce, dim3((ns + 255) / 256), dim3(256), 0, st, (const double4 *)part, ns,
+                       out);
     return hipGetLastError();
 }
 

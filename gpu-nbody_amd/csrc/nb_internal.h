@@ -12,11 +12,14 @@
 //            start : int32[cap]     first sorted position of the subtree
 //            com64 : double4[cap]   FP64 centre of mass (x, y, z) and mass (w)
 //            cchild: int32[8*cap]   children compacted to the front; cells as
-//                                   N + cell, bodies as their sorted position
+//                                   N + cell, bodies by index (-1 after the last)
+//            cdata : float4[8*cap]  (x, y, z, m) of those children: body position
+//                                   and mass, or the child cell's FP32 centre of
+//                                   mass and mass -- a popped cell is one
+//                                   contiguous 160-byte read in the traversal
 //   levels   lvl_list : int32[cap]  live cells grouped by level
-//   traversal nodes4 : float4[N+cap] (x, y, z, m): bodies in sorted order at
-//                                   [0, N), centres of mass of cells at N + cell
-//            sorted : int32[N]     body index at each sorted position
+//   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
+//            sorted : int32[N]      body index at each sorted position
 //
 // Body records are 16-byte AoS: streaming kernels still read them with fully
 // coalesced 16-byte lanes, and the gathers of the tree stages touch one cache
@@ -79,10 +82,12 @@ struct Dev {
     int *count, *start;
     double4 *com64;
     int *cchild;
+    float4 *cdata;
     int *lvl_list;
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
-    float4 *nodes4;
+    float4 *bnode;
     int *sorted;
+    float4 *probe;      // accelerations of a probe force pass (error checks)
     float *bbox_part; // 6 * kBBoxBlocks
 };
 
@@ -92,7 +97,11 @@ hipError_t launch_build(const Dev &d, int blocks, hipStream_t st);
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
 hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st);
 hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st);
-hipError_t launch_force(const Dev &d, int vote, int fused, hipStream_t st);
+// mode: 0 = store accelerations (reference semantics), 1 = also integrate
+// (fused kick-drift-kick), 2 = probe: write accelerations to d.probe only
+hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);
+// FP64 direct-sum accelerations of the bodies idx[0..ns) (G = 1), into out
+hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, hipStream_t st);
 hipError_t launch_integrate(const Dev &d, hipStream_t st);
 hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st);
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st);

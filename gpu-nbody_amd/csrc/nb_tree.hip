@@ -411,9 +411,11 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     for (int k = 0; k < 8; ++k) {
         const int v = ch[k];
         if (v < 0) continue;
+        float4 *cd = d.cdata + (size_t)c * 8 + used;
         out[used++] = v;
         if (v < n) {
             const float4 b = d.pm4[v];
+            *cd = b;
             const double mb = (double)b.w;
             m += mb;
             sx = fma(mb, (double)b.x, sx);
@@ -422,6 +424,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             cnt += 1;
         } else {
             const double4 q = d.com64[v - n];
+            *cd = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
             m += q.w;
             sx = fma(q.w, q.x, sx);
             sy = fma(q.w, q.y, sy);
@@ -429,13 +432,15 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             cnt += d.count[v - n];
         }
     }
-    for (int k = used; k < 8; ++k) out[k] = -1;
+    for (int k = used; k < 8; ++k) {
+        out[k] = -1;
+        d.cdata[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     // a cell of zero mass gets its centre at the origin instead of the
     // reference's 0 * inf = NaN (summarizetree.cl:161-167)
     const double inv = m > 0.0 ? 1.0 / m : 0.0;
     const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
     d.com64[c] = com;
-    d.nodes4[n + c] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)m);
     d.count[c] = cnt;
 }
 
@@ -443,9 +448,8 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
 // Stage 4: sort (sort.cl:12-76), one launch per level from the root down. A
 // cell's start arrived from its parent in the previous launch; it hands
 // consecutive offsets to its children in octant order: child cells get their
-// start, bodies their sorted position, their (x, y, z, m) in sorted order, and
-// the traversal list is rewritten to reference that position. The result is
-// the octant-path (Morton) order of the bodies.
+// start, bodies their sorted position and their (x, y, z, m) in sorted order.
+// The result is the octant-path (Morton) order of the bodies.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
 {
@@ -456,7 +460,7 @@ __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
     const int c = d.lvl_list[ctl->lvl_start[level] + i];
     int st = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
     if (level == 0) d.start[c] = 0;
-    int *cc = d.cchild + (size_t)c * 8;
+    const int *cc = d.cchild + (size_t)c * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int v = cc[k];
@@ -467,8 +471,7 @@ __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
         } else { // sort.cl:238-245
             if (st < 0 || st >= n) { set_error(ctl, kErrSync); return; }
             d.sorted[st] = v;
-            d.nodes4[st] = d.pm4[v];
-            cc[k] = st;
+            d.bnode[st] = d.cdata[(size_t)c * 8 + k];
             ++st;
         }
     }

@@ -108,6 +108,9 @@ def load() -> C.CDLL:
     L.nb_get_cells.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), vp, vp, vp, vp, vp]
     L.nb_get_counters.argtypes = [vp, C.POINTER(NbCounters)]
     L.nb_set_timing.argtypes = [vp, C.c_int32]
+    L.nb_probe_accel.argtypes = [vp, vp, vp, vp]
+    L.nb_direct_accel.argtypes = [vp, _i32p, C.c_int32, vp]
+    L.nb_force_error.argtypes = [vp, C.c_int32, C.c_uint64, vp]
     L.nb_last_error.argtypes = [vp]
     L.nb_last_error.restype = C.c_char_p
     L.nb_status_string.argtypes = [C.c_int]
@@ -115,7 +118,7 @@ def load() -> C.CDLL:
     for name in ("nb_create", "nb_set_state", "nb_step", "nb_run_stage", "nb_copy_vertices",
                  "nb_get_state", "nb_get_accel", "nb_get_positions64", "nb_get_root",
                  "nb_get_sorted", "nb_get_paths", "nb_get_cells", "nb_get_counters",
-                 "nb_set_timing"):
+                 "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

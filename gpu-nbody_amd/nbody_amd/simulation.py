@@ -237,5 +237,25 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
         _check(self._lib, self._h, self._lib.nb_get_counters(self._h, C.byref(c)))
         return c
 
+    def probe_accel(self) -> np.ndarray:
+        """Accelerations of the configured force method at the current positions."""
+        n = self.nbodies
+        a = [np.zeros(n, np.float32) for _ in range(3)]
+        _check(self._lib, self._h, self._lib.nb_probe_accel(self._h, *[ptr(x) for x in a]))
+        return np.stack(a, axis=1)
+
+    def direct_accel(self, idx) -> np.ndarray:
+        """FP64 direct-sum accelerations of the bodies in idx (computed on the GPU)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((len(idx), 3), np.float64)
+        _check(self._lib, self._h, self._lib.nb_direct_accel(self._h, idx, len(idx), ptr(out)))
+        return out
+
+    def force_error(self, nsamples: int = 4096, seed: int = 1000) -> dict:
+        """Relative force error against the FP64 direct sum on sampled bodies."""
+        st = np.zeros(4, np.float64)
+        _check(self._lib, self._h, self._lib.nb_force_error(self._h, int(nsamples), int(seed), ptr(st)))
+        return dict(median=st[0], p90=st[1], max=st[2], mean=st[3])
+
     def set_timing(self, on: bool) -> None:
         _check(self._lib, self._h, self._lib.nb_set_timing(self._h, 1 if on else 0))

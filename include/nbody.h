@@ -153,12 +153,29 @@ typedef struct nb_counters {
     int64_t interactions;  /* body-node interactions of the last force stage */
     int64_t error;         /* device error word */
     int64_t fallbacks;     /* learned mode: steps that fell back to plain theta */
-    double stage_ms[8];    /* last step's per-stage device time (when timing enabled) */
+    double stage_ms[8];    /* per-step device time of each stage over the last nb_step call (timing on) */
     double last_check_median; /* learned mode: last sampled median relative error */
     int64_t live_cells;    /* cells linked into the tree by the last build */
     int64_t reserved[5];
 } nb_counters;
 int nb_get_counters(nb_sim *sim, nb_counters *out);
+
+/* ---- force accuracy (no reference counterpart: the reference has no error
+ *      measurement; these back the sampled FP64 check of the learned mode and
+ *      the error figures of the benchmark) ---------------------------------- */
+
+/* Accelerations that the configured force method gives for the current
+ * positions (bounding box .. force, without advancing the bodies). */
+int nb_probe_accel(nb_sim *sim, float *ax, float *ay, float *az);
+
+/* FP64 direct-sum accelerations (G = 1, same softening) of the bodies idx[0..ns)
+ * at the current positions, computed on the device; out3 holds 3*ns doubles. */
+int nb_direct_accel(nb_sim *sim, const int32_t *idx, int32_t ns, double *out3);
+
+/* Relative force error of the configured method against the FP64 direct sum
+ * on nsamples distinct bodies drawn with `seed`: stats4 = (median, 90th
+ * percentile, maximum, mean) of |a - a_direct| / |a_direct|. */
+int nb_force_error(nb_sim *sim, int32_t nsamples, uint64_t seed, double *stats4);
 
 /* Per-stage HIP-event timing of the following nb_step calls (0 = off). */
 int nb_set_timing(nb_sim *sim, int32_t enabled);

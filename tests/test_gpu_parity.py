@@ -202,3 +202,38 @@ def test_coincident_bodies_report_error(nb):
         sim.steps(1)
     assert e.value.code in (-2, -3)
     sim.close()
+
+
+def test_direct_sum_matches_oracle(nb, oracle):
+    """The device FP64 direct sum (the error yardstick) against the oracle's."""
+    st = _universes()["clusters_32k"]()
+    sim = make_sim(st, **PHYS)
+    idx = np.arange(0, 32768, 97, dtype=np.int32)
+    g = sim.direct_accel(idx)
+    o = oracle.direct_fp64(st[0], st[1], st[2], st[6], 0.05, idx)
+    assert np.max(np.abs(g - o) / np.linalg.norm(o, axis=1)[:, None]) < 1e-12
+    sim.close()
+
+
+@pytest.mark.parametrize("vote", [16, 64])
+def test_probe_and_error(nb, oracle, vote):
+    """The probe pass gives the accelerations of a force stage on the same
+    positions without advancing the bodies; theta -> 0 has only the FP32
+    rounding error against the FP64 direct sum."""
+    st = _universes()["plummer_32k"]()
+    sim = make_sim(st, vote_width=vote, **PHYS)
+    probe = sim.probe_accel()
+    assert sim.counters().step == -1
+    assert np.array_equal(sim.state()[0], st[0])
+    for s in range(1, 6):
+        sim.run_stage(s)
+    assert np.array_equal(probe, sim.accel())
+    sim.close()
+    sim = make_sim(st, vote_width=vote, theta=1e-4, softening=0.05, dt=1e-3)
+    e = sim.force_error(512, seed=3)
+    assert e["max"] < 2e-5 and e["median"] < 2e-6
+    sim.close()
+    sim = make_sim(st, vote_width=vote, theta=0.7, softening=0.05, dt=1e-3)
+    e = sim.force_error(2048, seed=3)
+    assert 1e-5 < e["median"] < 1e-2
+    sim.close()
