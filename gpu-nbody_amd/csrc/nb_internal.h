@@ -17,6 +17,9 @@
 //                                   and mass, or the child cell's FP32 centre of
 //                                   mass and mass -- a popped cell is one
 //                                   contiguous 160-byte read in the traversal
+//            cq64  : double[6*cap]  traceless quadrupole about the centre of mass
+//                                   (xx, yy, zz, xy, xz, yz), learned mode only
+//            cqa/cqb: float4/float2[8*cap] the children's quadrupoles, next to cdata
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
 //            sorted : int32[N]      body index at each sorted position
@@ -62,6 +65,7 @@ struct Ctl {
     float radius;        // reference _radius
     float root[3];
     float dq[kMaxDepth]; // opening thresholds (calculateforce.cl:52-67)
+    float dq_coarse[kMaxDepth]; // the same for the coarse opening angle (learned mode)
     unsigned long long interactions; // cumulative body-node interactions
     int levels;          // number of non-empty cell levels of the last tree
     int live_cells;      // cells linked into the tree by the last build
@@ -73,7 +77,9 @@ struct Dev {
     int n;
     int cap; // cells in the pool
     float theta2, eps2, dt;
+    float theta2_coarse; // learned mode: coarse opening angle squared
     int fp64;
+    int quad;            // summarize also computes quadrupole moments
     Ctl *ctl;
     float4 *pm4, *vel4, *acc4;
     double4 *pos64;
@@ -83,6 +89,9 @@ struct Dev {
     double4 *com64;
     int *cchild;
     float4 *cdata;
+    double *cq64;
+    float4 *cqa;
+    float2 *cqb;
     int *lvl_list;
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
     float4 *bnode;

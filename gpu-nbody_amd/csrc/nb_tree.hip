@@ -136,11 +136,14 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
         c->radius = radius;
         // opening thresholds, calculateforce.cl:52-67 (FP32 quotient via a
         // double division rounded once = the correctly rounded FP32 quotient)
-        float raw = radius * radius;
+        float raw = radius * radius, rawc = radius * radius;
         if (d.theta2 > 0.0f) raw = (float)((double)raw / (double)d.theta2);
+        if (d.theta2_coarse > 0.0f) rawc = (float)((double)rawc / (double)d.theta2_coarse);
         for (int i = 0; i < kMaxDepth; ++i) {
             c->dq[i] = raw + d.eps2;
+            c->dq_coarse[i] = rawc + d.eps2;
             raw = 0.25f * raw;
+            rawc = 0.25f * rawc;
         }
         c->n_cells = 1; // the root is cell 0 (reference: _bottom = NUMBER_OF_NODES, :180)
         c->epoch += 1;  // every child slot of the last tree turns stale
@@ -154,27 +157,35 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
 //
 // A child slot is a 64-bit word (epoch << 32 | child); a word of another
 // epoch reads as empty, so the previous step's tree needs no clearing. Slot
-// states only move forward within a step: empty -> body -> cell. A body goes
-// into an empty slot with one CAS. A slot that holds a body d is split by
-// taking a cell c from the pool, placing d in c's slot for its octant, and
-// publishing c with a release CAS of d -> N+c; the inserting body then
-// continues inside c. A lost race leaves c unpublished: it is cleared and kept
-// as the thread's spare for its next split. Cell centres are never read from
-// memory: each thread derives them from the root along its own path with the
-// reference's arithmetic (buildtree.cl:124-136).
+// states only move forward within a step: empty -> body -> cell, and every
+// change is a CAS from the value the thread read to the new one:
+//   * a body goes into an empty slot with one CAS;
+//   * a slot that holds a body d is split by publishing a fresh cell c in it
+//     (CAS d -> N+c); d is now out of the tree and the thread re-inserts it,
+//     starting at c, before it goes on with its own body (a small per-thread
+//     stack in LDS keeps the bodies still to be placed). c is published with
+//     no child yet, so no other write has to become visible before it: no
+//     release fence is needed (on MI355X one costs an L2 write-back).
+// A lost race leaves c unpublished and untouched; the thread keeps it as a
+// spare for its next split, so at most one cell per thread is ever unused.
+// Cell centres are never read from memory: each thread derives them from the
+// root along its own path with the reference's arithmetic
+// (buildtree.cl:124-136), which is what makes the tree bit-identical.
 //
-// Every decision to write a slot is a CAS against the value the thread read,
-// and the CAS compares with the current value; a slot that holds a cell never
-// changes again. A stale read therefore costs at most a failed CAS, so the
-// descent reads slots with ordinary cached loads and keeps, per thread, the
-// cells of its previous path in LDS: the next body of the chunk (its octant-
-// order neighbour) skips the levels the two paths share.
+// Because a CAS compares with the current value and a slot holding a cell
+// never changes again, a stale read costs at most a failed CAS: descents read
+// slots with ordinary cached loads, and each thread keeps the cells of its
+// previous path in LDS so that the next body of its chunk (its octant-order
+// neighbour) skips the levels the two paths share.
 //
 // Insertion order: thread t inserts a contiguous chunk of the previous step's
 // sorted (octant-path) order, so consecutive insertions of a thread share most
-// of their path and concurrent threads work in different regions.
+// of their path and concurrent threads work in different regions. A first
+// launch with a single block inserts an evenly spaced sample of that order,
+// so the top levels exist before all threads start.
 // ---------------------------------------------------------------------------
 constexpr int kPathCache = 24; // levels of the previous path kept per thread
+constexpr int kPending = 8;    // bodies waiting to be re-inserted, per thread
 
 __device__ __forceinline__ int path_oct(unsigned long long w0, unsigned long long w1,
                                         unsigned long long w2, int level)
@@ -206,9 +217,13 @@ __device__ __forceinline__ int alloc_cell(Ctl *ctl)
     return base + rank;
 }
 
-__global__ __launch_bounds__(256) void k_build(Dev d)
+// seed: insert sorted[k * stride] for k < nseed; otherwise every other body,
+// a contiguous chunk of the sorted order per thread
+__global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int seed)
 {
     __shared__ int pc[kPathCache][256];
+    __shared__ int pend_body[kPending][256];
+    __shared__ unsigned pend_cell[kPending][256]; // cell << 6 | level
     const int n = d.n, cap = d.cap;
     Ctl *ctl = d.ctl;
     if (ctl->error) return;
@@ -216,18 +231,34 @@ __global__ __launch_bounds__(256) void k_build(Dev d)
     const float radius = ctl->radius;
     const float rx = ctl->root[0], ry = ctl->root[1], rz = ctl->root[2];
     const int tid = threadIdx.x;
+    const int gtid = blockIdx.x * blockDim.x + tid;
     const int T = gridDim.x * blockDim.x;
-    const int chunk = (n + T - 1) / T;
-    const int first = (blockIdx.x * blockDim.x + tid) * chunk;
-    const int last = min(n, first + chunk);
+    int first, last, step;
+    if (seed) {
+        first = gtid;
+        last = nseed;
+        step = T;
+    } else {
+        const int chunk = (n + T - 1) / T;
+        first = gtid * chunk;
+        last = min(n, first + chunk);
+        step = 1;
+    }
     int my_max = 0;
     int spare = -1;
     bool failed = false;
     unsigned long long po0 = 0, po1 = 0, po2 = 0; // octants of the cached path
     int plen = 0; // pc[1 .. plen-1] hold the cells of the cached path
-    for (int i = first; i < last && !failed; ++i) {
-        const int b = d.sorted[i];
-        const float4 me = d.pm4[b];
+    for (int i = first; i < last && !failed; i += step) {
+        int b;
+        if (seed) {
+            b = d.sorted[i * stride];
+        } else {
+            if (i % stride == 0 && i / stride < nseed) continue; // placed by the seed launch
+            b = d.sorted[i];
+        }
+        int cur = b, npend = 0;
+        float4 me = d.pm4[cur];
         int p = 0, level = 0;
         float cx = rx, cy = ry, cz = rz, r = radius;
         // follow the previous path while the octants agree
@@ -243,62 +274,86 @@ __global__ __launch_bounds__(256) void k_build(Dev d)
             ++level;
         }
         plen = level + 1;
-        for (;;) {
-            const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
-            unsigned long long *sp = d.slot + (size_t)p * 8 + oct;
-            unsigned long long w = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            int v = dec_slot(w, ep);
-            int next = -1; // cell to descend into
-            const float rh = r * 0.5f;
-            const float ncx = (cx - rh) + ((oct & 1) ? r : 0.0f); // buildtree.cl:124-136
-            const float ncy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-            const float ncz = (cz - rh) + ((oct & 4) ? r : 0.0f);
-            for (int tries = 0;; ++tries) {
-                if (tries > 16) { set_error(ctl, kErrSync); failed = true; break; }
-                if (v >= n) { next = v - n; break; }
-                if (v < 0) {
-                    if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, b), __ATOMIC_RELAXED,
+        for (;;) { // place `cur`, starting in cell p at `level`
+            bool placed = false;
+            for (;;) {
+                const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
+                unsigned long long *sp = d.slot + (size_t)p * 8 + oct;
+                unsigned long long w = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                int v = dec_slot(w, ep);
+                int next = -1; // cell to descend into
+                for (int tries = 0;; ++tries) {
+                    if (tries > 16) { set_error(ctl, kErrSync); failed = true; break; }
+                    if (v >= n) { next = v - n; break; }
+                    if (v < 0) {
+                        if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, cur), __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, AGENT)) {
+                            placed = true;
+                            break;
+                        }
+                        v = dec_slot(w, ep);
+                        continue;
+                    }
+                    // slot holds body v: split it (buildtree.cl:102-181)
+                    if (level + 1 >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
+                    int c = spare;
+                    if (c < 0) c = alloc_cell(ctl);
+                    spare = -1;
+                    if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
+                    if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, AGENT)) {
-                        my_max = max(my_max, level + 1);
+                        d.cinfo[c] = make_int2(p, (level + 1) | (oct << 8));
+                        // cur goes on later from c; the displaced body v first
+                        if (npend >= kPending) { set_error(ctl, kErrSync); failed = true; break; }
+                        pend_body[npend][tid] = cur;
+                        pend_cell[npend][tid] = ((unsigned)c << 6) | (unsigned)(level + 1);
+                        ++npend;
+                        cur = v;
+                        me = d.pm4[cur];
+                        next = c;
                         break;
                     }
+                    spare = c;
                     v = dec_slot(w, ep);
-                    continue;
                 }
-                // slot holds body v: split it (buildtree.cl:102-181)
-                if (level + 1 >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
-                int c = spare;
-                if (c < 0) c = alloc_cell(ctl);
-                spare = -1;
-                if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
-                const float4 dv = d.pm4[v];
-                const int od = octant(ncx, ncy, ncz, dv.x, dv.y, dv.z);
-                unsigned long long *csl = d.slot + (size_t)c * 8 + od;
-                __hip_atomic_store(csl, tag(ep, v), __ATOMIC_RELAXED, AGENT);
-                if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELEASE,
-                                                         __ATOMIC_RELAXED, AGENT)) {
-                    d.cinfo[c] = make_int2(p, (level + 1) | (oct << 8));
-                    my_max = max(my_max, level + 2);
-                    next = c;
+                if (failed || placed) {
+                    if (placed) my_max = max(my_max, level + 1);
                     break;
                 }
-                __hip_atomic_store(csl, tag(ep, (int)kEmpty), __ATOMIC_RELAXED, AGENT);
-                spare = c;
-                v = dec_slot(w, ep);
+                if (next >= cap) { set_error(ctl, kErrSync); failed = true; break; }
+                // descend (buildtree.cl:277-289), remembering the path
+                set_path_oct(po0, po1, po2, level, oct);
+                if (level + 1 < kPathCache) {
+                    pc[level + 1][tid] = next;
+                    plen = level + 2;
+                }
+                const float rh = r * 0.5f;
+                cx = (cx - rh) + ((oct & 1) ? r : 0.0f); // buildtree.cl:124-136
+                cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
+                cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
+                r = rh;
+                p = next;
+                ++level;
+                if (level >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
             }
-            if (failed || next < 0) break;
-            if (next >= cap) { set_error(ctl, kErrSync); failed = true; break; }
-            // descend (buildtree.cl:277-289), remembering the path
-            set_path_oct(po0, po1, po2, level, oct);
-            if (level + 1 < kPathCache) {
-                pc[level + 1][tid] = next;
-                plen = level + 2;
+            if (failed || npend == 0) break;
+            // next waiting body: it continues in the cell where it was displaced
+            --npend;
+            cur = pend_body[npend][tid];
+            const unsigned e = pend_cell[npend][tid];
+            p = (int)(e >> 6);
+            const int target = (int)(e & 63u);
+            me = d.pm4[cur];
+            cx = rx; cy = ry; cz = rz; r = radius;
+            for (level = 0; level < target; ++level) {
+                const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
+                const float rh = r * 0.5f;
+                cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
+                cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
+                cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
+                r = rh;
             }
-            cx = ncx; cy = ncy; cz = ncz;
-            r = rh;
-            p = next;
-            ++level;
-            if (level >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
+            plen = min(plen, level + 1);
         }
     }
     // running maximum depth (buildtree.cl:199)
@@ -442,6 +497,44 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
     d.com64[c] = com;
     d.count[c] = cnt;
+    if (d.quad) {
+        // traceless quadrupole about the centre of mass, parallel-axis rule:
+        // Q = sum_children Q_child + m_child (3 dd^T - |d|^2 I), d = child - com
+        double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < used; ++k) {
+            const int v = out[k];
+            double mx, px, py, pz;
+            float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
+            float2 qb = make_float2(0.f, 0.f);
+            if (v < n) {
+                const float4 b = d.pm4[v];
+                mx = b.w; px = b.x; py = b.y; pz = b.z;
+            } else {
+                const double4 cc = d.com64[v - n];
+                mx = cc.w; px = cc.x; py = cc.y; pz = cc.z;
+                const double *cq = d.cq64 + (size_t)(v - n) * 6;
+                for (int j = 0; j < 6; ++j) q[j] += cq[j];
+                qa = make_float4((float)cq[0], (float)cq[1], (float)cq[2], (float)cq[3]);
+                qb = make_float2((float)cq[4], (float)cq[5]);
+            }
+            d.cqa[(size_t)c * 8 + k] = qa;
+            d.cqb[(size_t)c * 8 + k] = qb;
+            const double dx = px - com.x, dy = py - com.y, dz = pz - com.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            q[0] += mx * (3.0 * dx * dx - r2);
+            q[1] += mx * (3.0 * dy * dy - r2);
+            q[2] += mx * (3.0 * dz * dz - r2);
+            q[3] += mx * 3.0 * dx * dy;
+            q[4] += mx * 3.0 * dx * dz;
+            q[5] += mx * 3.0 * dy * dz;
+        }
+        for (int k = used; k < 8; ++k) {
+            d.cqa[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            d.cqb[(size_t)c * 8 + k] = make_float2(0.f, 0.f);
+        }
+        double *cq = d.cq64 + (size_t)c * 6;
+        for (int j = 0; j < 6; ++j) cq[j] = q[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -562,7 +655,11 @@ hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st)
 }
 hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_build, dim3(blocks), dim3(256), 0, st, d);
+    // seed the top levels with one block, then every other body
+    const int nseed = d.n >= 65536 ? 4096 : 0;
+    const int stride = nseed ? d.n / nseed : 1;
+    if (nseed) hipLaunchKernelGGL(k_build, dim3(1), dim3(256), 0, st, d, stride, nseed, 1);
+    hipLaunchKernelGGL(k_build, dim3(blocks), dim3(256), 0, st, d, stride, nseed, 0);
     return hipGetLastError();
 }
 hipError_t launch_levels(const Dev &d, hipStream_t st)

@@ -709,3 +709,107 @@ void orc_set_threads(int t)
     (void)t;
 #endif
 }
+
+/* ---- MLP of the learned correction (no reference counterpart) ------------
+ * 24 -> 64 -> 64 -> 3, GELU (tanh form) on both hidden layers. Parameter
+ * layout (5955 values): W1[24][64], b1[64], W2[64][64], b2[64], W3[64][3],
+ * b3[3], W[i][j] connecting input i to output j. FP64 throughout: this is the
+ * yardstick for the engine's FP32 training kernels and BF16 inference. */
+#define ORC_MLP_IN 24
+#define ORC_MLP_H 64
+#define ORC_MLP_OUT 3
+#define ORC_MLP_PARAMS (ORC_MLP_IN * ORC_MLP_H + ORC_MLP_H + ORC_MLP_H * ORC_MLP_H + ORC_MLP_H + ORC_MLP_H * ORC_MLP_OUT + ORC_MLP_OUT)
+
+static double gelu(double x)
+{
+    const double c = 0.7978845608028654; /* sqrt(2/pi) */
+    return 0.5 * x * (1.0 + tanh(c * (x + 0.044715 * x * x * x)));
+}
+
+static double gelu_grad(double x)
+{
+    const double c = 0.7978845608028654;
+    const double u = c * (x + 0.044715 * x * x * x);
+    const double t = tanh(u);
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * c * (1.0 + 3.0 * 0.044715 * x * x);
+}
+
+int orc_mlp_params(void) { return ORC_MLP_PARAMS; }
+
+static void mlp_one(const double *P, const float *x, double *z1, double *h1, double *z2, double *h2, double *y)
+{
+    const double *W1 = P, *b1 = W1 + ORC_MLP_IN * ORC_MLP_H, *W2 = b1 + ORC_MLP_H;
+    const double *b2 = W2 + ORC_MLP_H * ORC_MLP_H, *W3 = b2 + ORC_MLP_H, *b3 = W3 + ORC_MLP_H * ORC_MLP_OUT;
+    for (int j = 0; j < ORC_MLP_H; ++j) {
+        double s = b1[j];
+        for (int i = 0; i < ORC_MLP_IN; ++i) s += (double)x[i] * W1[i * ORC_MLP_H + j];
+        z1[j] = s;
+        h1[j] = gelu(s);
+    }
+    for (int j = 0; j < ORC_MLP_H; ++j) {
+        double s = b2[j];
+        for (int i = 0; i < ORC_MLP_H; ++i) s += h1[i] * W2[i * ORC_MLP_H + j];
+        z2[j] = s;
+        h2[j] = gelu(s);
+    }
+    for (int j = 0; j < ORC_MLP_OUT; ++j) {
+        double s = b3[j];
+        for (int i = 0; i < ORC_MLP_H; ++i) s += h2[i] * W3[i * ORC_MLP_OUT + j];
+        y[j] = s;
+    }
+}
+
+/* y[n][3] = MLP(x[n][24]) */
+void orc_mlp_forward(const double *P, const float *x, int n, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; ++s) {
+        double z1[ORC_MLP_H], h1[ORC_MLP_H], z2[ORC_MLP_H], h2[ORC_MLP_H];
+        mlp_one(P, x + (size_t)s * ORC_MLP_IN, z1, h1, z2, h2, y + (size_t)s * ORC_MLP_OUT);
+    }
+}
+
+/* Mean squared error over n samples and its gradient with respect to every
+ * parameter (backpropagation), FP64. Returns the loss. */
+double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n, double *grad)
+{
+    const double *W2 = P + ORC_MLP_IN * ORC_MLP_H + ORC_MLP_H;
+    const double *W3 = W2 + ORC_MLP_H * ORC_MLP_H + ORC_MLP_H;
+    for (int k = 0; k < ORC_MLP_PARAMS; ++k) grad[k] = 0.0;
+    double *gW1 = grad, *gb1 = gW1 + ORC_MLP_IN * ORC_MLP_H, *gW2 = gb1 + ORC_MLP_H;
+    double *gb2 = gW2 + ORC_MLP_H * ORC_MLP_H, *gW3 = gb2 + ORC_MLP_H, *gb3 = gW3 + ORC_MLP_H * ORC_MLP_OUT;
+    double loss = 0.0;
+    for (int s = 0; s < n; ++s) {
+        const float *xs = x + (size_t)s * ORC_MLP_IN;
+        double z1[ORC_MLP_H], h1[ORC_MLP_H], z2[ORC_MLP_H], h2[ORC_MLP_H], y[ORC_MLP_OUT];
+        mlp_one(P, xs, z1, h1, z2, h2, y);
+        double dy[ORC_MLP_OUT], dz2[ORC_MLP_H], dz1[ORC_MLP_H];
+        for (int j = 0; j < ORC_MLP_OUT; ++j) {
+            const double e = y[j] - (double)t[(size_t)s * ORC_MLP_OUT + j];
+            loss += e * e;
+            dy[j] = 2.0 * e / ((double)n * ORC_MLP_OUT);
+            gb3[j] += dy[j];
+        }
+        for (int i = 0; i < ORC_MLP_H; ++i) {
+            double dh = 0.0;
+            for (int j = 0; j < ORC_MLP_OUT; ++j) {
+                gW3[i * ORC_MLP_OUT + j] += h2[i] * dy[j];
+                dh += W3[i * ORC_MLP_OUT + j] * dy[j];
+            }
+            dz2[i] = dh * gelu_grad(z2[i]);
+            gb2[i] += dz2[i];
+        }
+        for (int i = 0; i < ORC_MLP_H; ++i) {
+            double dh = 0.0;
+            for (int j = 0; j < ORC_MLP_H; ++j) {
+                gW2[i * ORC_MLP_H + j] += h1[i] * dz2[j];
+                dh += W2[i * ORC_MLP_H + j] * dz2[j];
+            }
+            dz1[i] = dh * gelu_grad(z1[i]);
+            gb1[i] += dz1[i];
+        }
+        for (int i = 0; i < ORC_MLP_IN; ++i)
+            for (int j = 0; j < ORC_MLP_H; ++j) gW1[i * ORC_MLP_H + j] += (double)xs[i] * dz1[j];
+    }
+    return loss / ((double)n * ORC_MLP_OUT);
+}

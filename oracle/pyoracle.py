@@ -105,11 +105,13 @@ def dq_table(radius, theta, softening, levels):
 
 
 def direct_fp64(x, y, z, m, softening, idx):
-    """FP64 direct-sum accelerations (G=1) on the bodies listed in idx."""
+    """FP64 direct-sum accelerations (G=1) on the bodies listed in idx. The
+    squared softening is the FP32 value softening*softening, as the force
+    kernels use it (calculateforce.cl:14: EPSILON is an FP32 constant)."""
     idx = np.ascontiguousarray(idx, dtype=np.int32)
     ax = np.zeros(len(idx)); ay = np.zeros(len(idx)); az = np.zeros(len(idx))
-    lib().orc_direct_fp64(len(x), _f32(x), _f32(y), _f32(z), _f32(m),
-                          float(softening) ** 2 if softening >= 0 else 0.0,
+    eps2 = float(np.float32(softening) * np.float32(softening))
+    lib().orc_direct_fp64(len(x), _f32(x), _f32(y), _f32(z), _f32(m), eps2,
                           len(idx), idx, ax, ay, az)
     return np.stack([ax, ay, az], axis=1)
 

@@ -223,7 +223,7 @@ def test_direct_fp64_bruteforce(oracle):
     idx = np.arange(0, 500, 7)
     got = oracle.direct_fp64(st[0], st[1], st[2], st[6], 0.05, idx)
     d = np.stack([x[None, :] - x[idx, None], y[None, :] - y[idx, None], z[None, :] - z[idx, None]], -1)
-    r2 = (d ** 2).sum(-1) + 0.05 ** 2
+    r2 = (d ** 2).sum(-1) + float(F(0.05) * F(0.05))
     exp = (d * (m[None, :] / r2 ** 1.5)[..., None]).sum(1)
     assert np.allclose(got, exp, rtol=1e-12, atol=1e-15)
 
