@@ -53,7 +53,7 @@ def make_sim(state, mode, model=None):
     kw = dict(theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote_width=VOTE,
               flags=_lib.NB_FP64_POSITIONS | _lib.NB_FUSED_UPDATE)
     if mode == "learned":
-        kw.update(mode=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE)
+        kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE)
     sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
     sim.init(None)
     if mode == "learned":

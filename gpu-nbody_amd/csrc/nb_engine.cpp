@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -38,6 +40,16 @@ struct nb_sim {
     int chk_cap = 0;
     int levels = 0;                      // non-empty cell levels of the current tree
     int lvl_start[nb::kMaxDepth + 2] = {}; // host copy of the level list offsets
+    // learned mode
+    nb::Learn L{};
+    void *mlpbuf = nullptr;      // packed inference network (nb::kMlpBufBytes)
+    std::vector<float> model;    // installed model blob (NB_MODEL_FLOATS)
+    bool have_model = false;
+    bool fell_back = false;      // a sampled check failed: plain theta until a new model
+    long long steps_since_model = 0;
+    uint64_t check_seed = 0x5eed;
+    float4 *chk_acc = nullptr;
+    hipEvent_t mid_ev = nullptr; // timing: between the coarse traversal and the network
     std::string err;
 };
 
@@ -88,8 +100,11 @@ int validate(const nb_params *p)
     if (p->vote_width != 16 && p->vote_width != 32 && p->vote_width != 64) return NB_E_ARG;
     if (p->flags & ~(NB_FP64_POSITIONS | NB_FUSED_UPDATE)) return NB_E_ARG;
     if (!(p->pool_factor >= 1.0f) || p->pool_factor > 64.0f) return NB_E_ARG;
-    if (p->mode != NB_MODE_PLAIN && p->mode != NB_MODE_LEARNED) return NB_E_ARG;
-    if (p->mode == NB_MODE_LEARNED && !(p->coarse_theta >= p->theta)) return NB_E_ARG;
+    if (p->mode != NB_MODE_PLAIN && p->mode != NB_MODE_LEARNED && p->mode != NB_MODE_COARSE) return NB_E_ARG;
+    if (!(p->coarse_theta > 0.0f) || !std::isfinite(p->coarse_theta)) return NB_E_ARG;
+    // the coarse traversal is wave-uniform: one vote group is one wave64
+    if (p->mode != NB_MODE_PLAIN && (p->vote_width != 64 || !(p->coarse_theta >= p->theta))) return NB_E_ARG;
+    if (!(p->check_tol >= 0.0f) || p->check_every < 0 || p->check_samples < 0) return NB_E_ARG;
     // the traversal stack packs (cell << 6 | depth) into 32 bits
     if ((double)pool_cells(p) > (double)(1 << 26)) return NB_E_ARG;
     return NB_OK;
@@ -152,6 +167,45 @@ int prepare_levels(nb_sim *s)
     return NB_OK;
 }
 
+// Device address of the coarse opening table inside Ctl.
+const float *dq_coarse_ptr(const nb_sim *s)
+{
+    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(s->d.ctl) + offsetof(nb::Ctl, dq_coarse));
+}
+const float *dq_plain_ptr(const nb_sim *s)
+{
+    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(s->d.ctl) + offsetof(nb::Ctl, dq));
+}
+
+// The force method in effect: the configured mode, or plain theta after a
+// failed sampled check.
+int force_method(const nb_sim *s) { return s->fell_back ? NB_MODE_PLAIN : s->p.mode; }
+
+// Force stage with an explicit method; fmode as nb::launch_force (0 store,
+// 1 fused update, 2 probe).
+int launch_force_method(nb_sim *s, int method, int fmode)
+{
+    const nb::Dev &d = s->d;
+    hipError_t e = hipSuccess;
+    switch (method) {
+    case NB_MODE_PLAIN: e = nb::launch_force(d, s->p.vote_width, fmode, s->stream); break;
+    case NB_MODE_COARSE:
+        e = nb::launch_coarse(d, s->L, nb::kOutFinish, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
+                              fmode, s->stream);
+        break;
+    case NB_MODE_LEARNED:
+        if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
+        e = nb::launch_coarse(d, s->L, nb::kOutInfer, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
+                              fmode, s->stream);
+        if (e == hipSuccess && s->timing && s->mid_ev) e = hipEventRecord(s->mid_ev, s->stream);
+        if (e == hipSuccess) e = nb::launch_mlp(d, s->L, s->L.feat, d.n, 0, nullptr, fmode, s->stream);
+        break;
+    default: return fail(s, NB_E_ARG, "unknown force method %d", method);
+    }
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "force launch: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
 int launch_stage(nb_sim *s, int stage)
 {
     const nb::Dev &d = s->d;
@@ -174,7 +228,7 @@ int launch_stage(nb_sim *s, int stage)
         for (int L = 0; L < s->levels && e == hipSuccess; ++L)
             e = nb::launch_sort_level(d, L, s->lvl_start[L + 1] - s->lvl_start[L], s->stream);
         break;
-    case NB_STAGE_FORCE: e = nb::launch_force(d, s->p.vote_width, fused, s->stream); break;
+    case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:
         if (!fused) e = nb::launch_integrate(d, s->stream);
         break;
@@ -184,17 +238,101 @@ int launch_stage(nb_sim *s, int stage)
     return NB_OK;
 }
 
+// Moment buffers of the quadrupole summarize (learned and coarse modes, training).
+int ensure_quad(nb_sim *s)
+{
+    nb::Dev &d = s->d;
+    const size_t cap = (size_t)d.cap;
+    if (!d.cq64) HIP_TRY(s, dalloc(&d.cq64, 8 * cap));
+    if (!d.cqa) HIP_TRY(s, dalloc(&d.cqa, 8 * cap));
+    if (!d.cqb) HIP_TRY(s, dalloc(&d.cqb, 8 * cap));
+    return NB_OK;
+}
+
+int ensure_learn(nb_sim *s)
+{
+    const size_t n = (size_t)s->d.n;
+    if (!s->L.feat) HIP_TRY(s, dalloc(&s->L.feat, n * nb::kFeat));
+    if (!s->L.frame) HIP_TRY(s, dalloc(&s->L.frame, 2 * n));
+    if (!s->mlpbuf) HIP_TRY(s, hipMalloc(&s->mlpbuf, nb::kMlpBufBytes));
+    return ensure_quad(s);
+}
+
+uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Validates a model blob, packs the inference network and uploads it.
+int install_model(nb_sim *s, const float *blob)
+{
+    for (int i = 0; i < NB_MODEL_FLOATS; ++i)
+        if (!std::isfinite(blob[i])) return fail(s, NB_E_MODEL, "model value %d is not finite", i);
+    const float *P = blob;
+    const float *xmu = blob + nb::kMlpParams, *xsd = xmu + nb::kFeat;
+    const float *ymu = xsd + nb::kFeat, *ysd = ymu + nb::kOut;
+    for (int i = 0; i < nb::kFeat; ++i)
+        if (!(xsd[i] > 0.0f)) return fail(s, NB_E_MODEL, "feature scale %d is not positive", i);
+    for (int i = 0; i < nb::kOut; ++i)
+        if (!(ysd[i] > 0.0f)) return fail(s, NB_E_MODEL, "target scale %d is not positive", i);
+    int r = ensure_learn(s);
+    if (r) return r;
+    std::vector<unsigned char> buf(nb::kMlpBufBytes, 0);
+    uint16_t *w1t = reinterpret_cast<uint16_t *>(buf.data());
+    uint16_t *w2t = w1t + 64 * 32;
+    uint16_t *w3t = w2t + 64 * 64;
+    float *fb = reinterpret_cast<float *>(w3t + 16 * 64);
+    float *b1 = fb, *b2 = fb + 64, *b3 = fb + 128, *mu = fb + 144, *rs = fb + 168;
+    for (int nn = 0; nn < 64; ++nn)
+        for (int k = 0; k < nb::kFeat; ++k) w1t[nn * 32 + k] = bf16_rne(P[k * 64 + nn]);
+    for (int nn = 0; nn < 64; ++nn)
+        for (int k = 0; k < 64; ++k) w2t[nn * 64 + k] = bf16_rne(P[nb::kOffW2 + k * 64 + nn]);
+    for (int nn = 0; nn < nb::kOut; ++nn)
+        for (int k = 0; k < 64; ++k) w3t[nn * 64 + k] = bf16_rne(P[nb::kOffW3 + k * 3 + nn] * ysd[nn]);
+    for (int i = 0; i < 64; ++i) {
+        b1[i] = P[nb::kOffB1 + i];
+        b2[i] = P[nb::kOffB2 + i];
+    }
+    for (int i = 0; i < nb::kOut; ++i) b3[i] = P[nb::kOffB3 + i] * ysd[i] + ymu[i];
+    for (int i = 0; i < nb::kFeat; ++i) {
+        mu[i] = xmu[i];
+        rs[i] = 1.0f / xsd[i];
+    }
+    HIP_TRY(s, hipMemcpy(s->mlpbuf, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    const unsigned char *base = static_cast<const unsigned char *>(s->mlpbuf);
+    const size_t fo = (size_t)(reinterpret_cast<unsigned char *>(fb) - buf.data());
+    s->L.mlp.w1t = reinterpret_cast<const uint16_t *>(base);
+    s->L.mlp.w2t = reinterpret_cast<const uint16_t *>(base) + 64 * 32;
+    s->L.mlp.w3t = reinterpret_cast<const uint16_t *>(base) + 64 * 32 + 64 * 64;
+    const float *dfb = reinterpret_cast<const float *>(base + fo);
+    s->L.mlp.b1 = dfb;
+    s->L.mlp.b2 = dfb + 64;
+    s->L.mlp.b3 = dfb + 128;
+    s->L.mlp.xmu = dfb + 144;
+    s->L.mlp.xrs = dfb + 168;
+    s->model.assign(blob, blob + NB_MODEL_FLOATS);
+    s->have_model = true;
+    s->fell_back = false;
+    s->steps_since_model = 0;
+    return NB_OK;
+}
+
 void free_all(nb_sim *s)
 {
     nb::Dev &d = s->d;
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
-                    s->chk_idx, s->chk_part, s->chk_out};
+                    s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb,
+                    s->L.feat, s->L.frame, s->mlpbuf, s->chk_acc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
         if (e) (void)hipEventDestroy(e);
+    if (s->mid_ev) (void)hipEventDestroy(s->mid_ev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -203,6 +341,112 @@ template <class T> int fetch(nb_sim *s, std::vector<T> &out, const T *src, size_
     out.resize(count);
     HIP_TRY(s, hipMemcpyAsync(out.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+int ensure_check_buffers(nb_sim *s, int ns)
+{
+    if (s->chk_cap >= ns) return NB_OK;
+    if (s->chk_idx) (void)hipFree(s->chk_idx);
+    if (s->chk_part) (void)hipFree(s->chk_part);
+    if (s->chk_out) (void)hipFree(s->chk_out);
+    s->chk_idx = nullptr;
+    s->chk_part = s->chk_out = nullptr;
+    s->chk_cap = 0;
+    HIP_TRY(s, hipMalloc((void **)&s->chk_idx, (size_t)ns * sizeof(int)));
+    HIP_TRY(s, hipMalloc((void **)&s->chk_part, (size_t)ns * 64 * sizeof(double4)));
+    HIP_TRY(s, hipMalloc((void **)&s->chk_out, (size_t)ns * sizeof(double4)));
+    if (s->chk_acc) (void)hipFree(s->chk_acc);
+    s->chk_acc = nullptr;
+    HIP_TRY(s, hipMalloc((void **)&s->chk_acc, (size_t)ns * sizeof(float4)));
+    s->chk_cap = ns;
+    return NB_OK;
+}
+
+// Device allocations that live for one API call; freed after the stream drains.
+struct Scratch {
+    nb_sim *s;
+    std::vector<void *> ptrs;
+    explicit Scratch(nb_sim *sim) : s(sim) {}
+    ~Scratch()
+    {
+        (void)hipStreamSynchronize(s->stream);
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T> int get(T **p, size_t count)
+    {
+        *p = nullptr;
+        hipError_t e = hipMalloc((void **)p, count * sizeof(T) + 64);
+        if (e != hipSuccess) return fail(s, NB_E_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+        ptrs.push_back(*p);
+        return NB_OK;
+    }
+};
+
+// Training buffers for S samples (X, T given or allocated) and batch B.
+int train_buffers(Scratch &m, nb::Train &t, int S, int B, int iters)
+{
+    int r = NB_OK;
+    for (float **f : {&t.P, &t.G, &t.M, &t.V})
+        if (!r) r = m.get(f, (size_t)nb::kMlpParams);
+    for (float **f : {&t.Z1, &t.H1, &t.Z2, &t.H2, &t.dZ1, &t.dZ2})
+        if (!r) r = m.get(f, (size_t)B * nb::kHid);
+    if (!r) r = m.get(&t.dY, (size_t)B * nb::kOut);
+    if (!r) r = m.get(&t.loss_part, (size_t)(B + 255) / 256);
+    if (!r) r = m.get(&t.loss_hist, (size_t)std::max(iters, 1));
+    t.S = S;
+    return r;
+}
+
+uint64_t splitmix64(uint64_t &x)
+{
+    x += 0x9e3779b97f4a7c15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Force stage of a checked learned step: the learned forces are stored
+// (velocity correction included), check_samples bodies drawn with replacement
+// are compared with the FP64 direct sum, and if the median relative error is
+// above check_tol the step's forces are recomputed with plain theta. The
+// second pass corrects the velocities by (a_plain - a_learned) dt / 2, so the
+// step ends exactly as a plain step would. The fallback holds until a new
+// model is installed.
+int checked_force(nb_sim *s)
+{
+    int r = launch_force_method(s, NB_MODE_LEARNED, 0);
+    if (r) return r;
+    const int n = s->d.n;
+    const int ns = std::min(s->p.check_samples, n);
+    r = ensure_check_buffers(s, ns);
+    if (r) return r;
+    std::vector<int32_t> idx((size_t)ns);
+    for (int i = 0; i < ns; ++i) idx[i] = (int32_t)(splitmix64(s->check_seed) % (uint64_t)n);
+    HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, s->stream);
+    if (e == hipSuccess) e = nb::launch_gather_acc(s->d, s->chk_idx, ns, s->chk_acc, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "sampled check launch: %s", hipGetErrorString(e));
+    std::vector<double4> ref;
+    std::vector<float4> got;
+    r = fetch(s, ref, (const double4 *)s->chk_out, (size_t)ns);
+    if (!r) r = fetch(s, got, (const float4 *)s->chk_acc, (size_t)ns);
+    if (r) return r;
+    std::vector<double> err((size_t)ns);
+    for (int i = 0; i < ns; ++i) {
+        const double dx = got[i].x - ref[i].x, dy = got[i].y - ref[i].y, dz = got[i].z - ref[i].z;
+        const double den = std::sqrt(ref[i].x * ref[i].x + ref[i].y * ref[i].y + ref[i].z * ref[i].z);
+        err[i] = std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(den, 1e-300);
+    }
+    std::nth_element(err.begin(), err.begin() + (ns - 1) / 2, err.end());
+    s->last_check_median = err[(size_t)(ns - 1) / 2];
+    if (s->last_check_median > (double)s->p.check_tol) {
+        r = launch_force_method(s, NB_MODE_PLAIN, 0);
+        if (r) return r;
+        ++s->fallbacks;
+        s->fell_back = true;
+    }
     return NB_OK;
 }
 
@@ -264,7 +508,9 @@ int nb_create(const nb_params *p, nb_sim **out)
     d.theta2 = p->theta * p->theta;       // THETA = theta^2
     d.eps2 = p->softening * p->softening; // EPSILON = softening^2
     d.dt = p->dt;
+    d.theta2_coarse = p->coarse_theta * p->coarse_theta;
     d.fp64 = (p->flags & NB_FP64_POSITIONS) ? 1 : 0;
+    d.quad = p->mode != NB_MODE_PLAIN ? 1 : 0;
     const size_t cap = (size_t)d.cap;
     bool ok = dalloc(&d.ctl, 1) == hipSuccess;
     for (float4 **f : {&d.pm4, &d.vel4, &d.acc4}) ok = ok && dalloc(f, (size_t)n) == hipSuccess;
@@ -284,6 +530,7 @@ int nb_create(const nb_params *p, nb_sim **out)
         fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
         return bail(NB_E_HIP);
     }
+    if (d.quad && ensure_learn(s) != NB_OK) return bail(NB_E_HIP);
     // epoch 0 tags every slot as stale; the first bounding box moves to epoch 1
     ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int2), s->stream) == hipSuccess;
@@ -367,6 +614,8 @@ int nb_step(nb_sim *s, int32_t nsteps)
     if (nsteps < 0) return fail(s, NB_E_ARG, "negative step count");
     if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
     HIP_TRY(s, hipSetDevice(s->device));
+    if (force_method(s) == NB_MODE_LEARNED && !s->have_model)
+        return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
     // with timing on, every stage of every step is bracketed by events on the
     // handle's stream; stage_ms reports the per-step average
     const size_t nev = s->timing ? (size_t)7 * nsteps : 0;
@@ -375,15 +624,39 @@ int nb_step(nb_sim *s, int32_t nsteps)
         HIP_TRY(s, hipEventCreate(&e));
         s->tev.push_back(e);
     }
+    std::vector<hipEvent_t> mids;
+    const bool learned = s->p.mode == NB_MODE_LEARNED;
     for (int k = 0; k < nsteps; ++k) {
         hipEvent_t *ev = s->timing ? s->tev.data() + 7 * (size_t)k : nullptr;
+        // the sampled FP64 check of the learned mode runs on every
+        // check_every-th step: force without the fused update, check, fall
+        // back to plain theta if the median error is above check_tol
+        const bool check = learned && !s->fell_back && s->p.check_every > 0 && s->p.check_tol > 0.0f &&
+                           s->p.check_samples > 0 && (s->steps_since_model % s->p.check_every) == 0;
+        if (ev && force_method(s) == NB_MODE_LEARNED) {
+            hipEvent_t m;
+            HIP_TRY(s, hipEventCreate(&m));
+            mids.push_back(m);
+            s->mid_ev = m;
+        } else {
+            s->mid_ev = nullptr;
+        }
         for (int st = NB_STAGE_BBOX; st <= NB_STAGE_INTEGRATE; ++st) {
             if (ev) HIP_TRY(s, hipEventRecord(ev[st - 1], s->stream));
-            int r = launch_stage(s, st);
+            int r;
+            if (check && st == NB_STAGE_FORCE)
+                r = checked_force(s);
+            else if (check && st == NB_STAGE_INTEGRATE)
+                r = nb::launch_integrate(s->d, s->stream) == hipSuccess ? NB_OK
+                                                                         : fail(s, NB_E_HIP, "integrate launch");
+            else
+                r = launch_stage(s, st);
             if (r) return r;
         }
         if (ev) HIP_TRY(s, hipEventRecord(ev[6], s->stream));
+        if (learned) ++s->steps_since_model;
     }
+    s->mid_ev = nullptr;
     int r = check_device_error(s);
     if (s->timing && nsteps > 0) {
         for (int st = 0; st < 6; ++st) {
@@ -395,7 +668,19 @@ int nb_step(nb_sim *s, int32_t nsteps)
             }
             s->stage_ms[st] = tot / nsteps;
         }
+        // learned mode: stage_ms[6] is the network, split off the force stage
+        double net = 0.0;
+        if (mids.size() == (size_t)nsteps) {
+            for (int k = 0; k < nsteps; ++k) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, mids[k], s->tev[7 * (size_t)k + 5]) == hipSuccess) net += ms;
+            }
+            net /= nsteps;
+        }
+        s->stage_ms[6] = net;
+        s->stage_ms[4] -= net;
     }
+    for (hipEvent_t m : mids) (void)hipEventDestroy(m);
     return r;
 }
 
@@ -565,26 +850,10 @@ static int probe_pass(nb_sim *s)
         r = launch_stage(s, st);
         if (r) return r;
     }
-    hipError_t e = nb::launch_force(s->d, s->p.vote_width, 2, s->stream);
-    if (e != hipSuccess) return fail(s, NB_E_HIP, "probe force launch: %s", hipGetErrorString(e));
+    r = launch_force_method(s, force_method(s), 2);
+    if (r) return r;
     HIP_TRY(s, hipMemcpyAsync(&s->d.ctl->step, &c0.step, sizeof(int), hipMemcpyHostToDevice, s->stream));
     return check_device_error(s);
-}
-
-static int ensure_check_buffers(nb_sim *s, int ns)
-{
-    if (s->chk_cap >= ns) return NB_OK;
-    if (s->chk_idx) (void)hipFree(s->chk_idx);
-    if (s->chk_part) (void)hipFree(s->chk_part);
-    if (s->chk_out) (void)hipFree(s->chk_out);
-    s->chk_idx = nullptr;
-    s->chk_part = s->chk_out = nullptr;
-    s->chk_cap = 0;
-    HIP_TRY(s, hipMalloc((void **)&s->chk_idx, (size_t)ns * sizeof(int)));
-    HIP_TRY(s, hipMalloc((void **)&s->chk_part, (size_t)ns * 64 * sizeof(double4)));
-    HIP_TRY(s, hipMalloc((void **)&s->chk_out, (size_t)ns * sizeof(double4)));
-    s->chk_cap = ns;
-    return NB_OK;
 }
 
 static int direct_accel(nb_sim *s, const int32_t *idx, int32_t ns, std::vector<double4> &out)
@@ -678,6 +947,311 @@ int nb_force_error(nb_sim *s, int32_t nsamples, uint64_t seed, double *stats4)
     stats4[3] = mean / ns;                            // mean
     s->last_check_median = stats4[0];
     return NB_OK;
+}
+
+// ---- learned coarse-theta correction ---------------------------------------
+
+void nb_default_train_params(nb_train_params *tp)
+{
+    if (!tp) return;
+    std::memset(tp, 0, sizeof *tp);
+    tp->steps = 50;
+    tp->groups_per_step = 512;
+    tp->iters = 4000;
+    tp->batch = 4096;
+    tp->lr = 2e-3f;
+    tp->lr_final = 1e-4f;
+    tp->beta1 = 0.9f;
+    tp->beta2 = 0.999f;
+    tp->adam_eps = 1e-8f;
+    tp->seed = 1;
+}
+
+int nb_set_model(nb_sim *s, const float *model, int32_t count)
+{
+    if (!s || !model) return NB_E_ARG;
+    if (count != NB_MODEL_FLOATS) return fail(s, NB_E_MODEL, "model blob has %d floats, expected %d", count, NB_MODEL_FLOATS);
+    HIP_TRY(s, hipSetDevice(s->device));
+    return install_model(s, model);
+}
+
+int nb_get_model(nb_sim *s, float *model, int32_t count)
+{
+    if (!s || !model) return NB_E_ARG;
+    if (!s->have_model) return fail(s, NB_E_MODEL, "no model installed");
+    if (count < NB_MODEL_FLOATS) return fail(s, NB_E_ARG, "model buffer holds %d of %d floats", count, NB_MODEL_FLOATS);
+    std::memcpy(model, s->model.data(), sizeof(float) * NB_MODEL_FLOATS);
+    return NB_OK;
+}
+
+int nb_train(nb_sim *s, const nb_train_params *tp_in, nb_train_report *rep)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    nb_train_params tp;
+    nb_default_train_params(&tp);
+    if (tp_in) tp = *tp_in;
+    if (tp.steps < 1 || tp.groups_per_step < 1 || tp.iters < 0 || tp.batch < 1 || !(tp.lr > 0.0f) ||
+        !(tp.lr_final >= 0.0f) || !(tp.beta1 >= 0.0f && tp.beta1 < 1.0f) || !(tp.beta2 >= 0.0f && tp.beta2 < 1.0f) ||
+        !(tp.adam_eps > 0.0f))
+        return fail(s, NB_E_ARG, "invalid training parameters");
+    if (s->p.vote_width != 64) return fail(s, NB_E_ARG, "training needs vote width 64");
+    nb::Dev &d = s->d;
+    const int ngroups_all = d.n / 64;
+    if (ngroups_all < 1) return fail(s, NB_E_ARG, "training needs at least 64 bodies");
+    const int G = std::min(tp.groups_per_step, ngroups_all);
+    const int rows = G * 64;
+    const long long S64 = (long long)tp.steps * rows;
+    if (S64 > (1LL << 26)) return fail(s, NB_E_ARG, "%lld training samples exceed 2^26", S64);
+    const int S = (int)S64;
+    const int B = std::min(tp.batch, S);
+    HIP_TRY(s, hipSetDevice(s->device));
+    int r = ensure_learn(s);
+    if (r) return r;
+    const int saved_quad = d.quad;
+    d.quad = 1;
+    struct QuadRestore {
+        nb::Dev &d;
+        int v;
+        ~QuadRestore() { d.quad = v; }
+    } restore{d, saved_quad};
+
+    Scratch m(s);
+    nb::Train t{};
+    float *X = nullptr, *T = nullptr, *stats = nullptr;
+    float4 *frame = nullptr, *afine = nullptr;
+    int *gdev = nullptr;
+    r = train_buffers(m, t, S, B, tp.iters);
+    if (!r) r = m.get(&X, (size_t)S * nb::kFeat);
+    if (!r) r = m.get(&T, (size_t)S * nb::kOut);
+    if (!r) r = m.get(&stats, (size_t)2 * (nb::kFeat + nb::kOut));
+    if (!r) r = m.get(&frame, (size_t)2 * rows);
+    if (!r) r = m.get(&afine, (size_t)rows);
+    if (!r) r = m.get(&gdev, (size_t)G * tp.steps);
+    if (r) return r;
+    t.X = X;
+    t.T = T;
+
+    // the vote groups of every step: G distinct groups each, drawn up front
+    uint64_t rng = tp.seed ^ 0x243f6a8885a308d3ull;
+    std::vector<int> perm((size_t)ngroups_all), groups((size_t)G * tp.steps);
+    for (int k = 0; k < tp.steps; ++k) {
+        for (int i = 0; i < ngroups_all; ++i) perm[i] = i;
+        for (int i = 0; i < G; ++i) std::swap(perm[i], perm[i + (int)(splitmix64(rng) % (uint64_t)(ngroups_all - i))]);
+        std::sort(perm.begin(), perm.begin() + G);
+        std::copy(perm.begin(), perm.begin() + G, groups.begin() + (size_t)k * G);
+    }
+    HIP_TRY(s, hipMemcpy(gdev, groups.data(), groups.size() * sizeof(int), hipMemcpyHostToDevice));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
+    for (int k = 0; k < tp.steps; ++k) {
+        for (int st = NB_STAGE_BBOX; st <= NB_STAGE_SORT; ++st) {
+            r = launch_stage(s, st);
+            if (r) return r;
+        }
+        hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, dq_coarse_ptr(s), gdev + (size_t)k * G, G,
+                                         X + (size_t)k * rows * nb::kFeat, frame, nullptr, 0, s->stream);
+        if (e == hipSuccess)
+            e = nb::launch_coarse(d, s->L, nb::kOutAccel, dq_plain_ptr(s), gdev + (size_t)k * G, G, nullptr,
+                                  nullptr, afine, 0, s->stream);
+        if (e == hipSuccess) e = nb::launch_make_target(frame, afine, rows, T + (size_t)k * rows * nb::kOut, s->stream);
+        if (e != hipSuccess) return fail(s, NB_E_HIP, "training pair launch: %s", hipGetErrorString(e));
+        // the bodies advance with plain theta
+        r = launch_force_method(s, NB_MODE_PLAIN, fused);
+        if (!r) r = launch_stage(s, NB_STAGE_INTEGRATE);
+        if (r) return r;
+    }
+    r = check_device_error(s);
+    if (r) return r;
+    const auto t1 = std::chrono::steady_clock::now();
+
+    float *xmu = stats, *xsd = stats + nb::kFeat, *ymu = stats + 2 * nb::kFeat, *ysd = ymu + nb::kOut;
+    hipError_t e = nb::launch_standardize(X, S, nb::kFeat, xmu, xsd, s->stream);
+    if (e == hipSuccess) e = nb::launch_standardize(T, S, nb::kOut, ymu, ysd, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "standardize launch: %s", hipGetErrorString(e));
+
+    // seeded initialization: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases
+    std::vector<float> P((size_t)nb::kMlpParams);
+    uint64_t irng = tp.seed * 0x9e3779b97f4a7c15ull + 7;
+    auto uni = [&](float bound) {
+        return (float)((double)(splitmix64(irng) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * bound;
+    };
+    const float bnd1 = 1.0f / std::sqrt((float)nb::kFeat), bnd2 = 1.0f / std::sqrt((float)nb::kHid);
+    for (int i = 0; i < nb::kOffB1; ++i) P[i] = uni(bnd1);
+    for (int i = nb::kOffB1; i < nb::kOffW2; ++i) P[i] = uni(bnd1);
+    for (int i = nb::kOffW2; i < nb::kMlpParams; ++i) P[i] = uni(bnd2);
+    HIP_TRY(s, hipMemcpyAsync(t.P, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemsetAsync(t.M, 0, (size_t)nb::kMlpParams * sizeof(float), s->stream));
+    HIP_TRY(s, hipMemsetAsync(t.V, 0, (size_t)nb::kMlpParams * sizeof(float), s->stream));
+    const uint64_t bseed = tp.seed * 2 + 1; // nonzero: random minibatches
+    double p1 = 1.0, p2 = 1.0;
+    for (int it = 0; it < tp.iters; ++it) {
+        const double frac = tp.iters > 1 ? (double)it / (tp.iters - 1) : 0.0;
+        const float lr = (float)(tp.lr_final + 0.5 * (tp.lr - tp.lr_final) * (1.0 + std::cos(M_PI * frac)));
+        p1 *= tp.beta1;
+        p2 *= tp.beta2;
+        e = nb::launch_train_iter(t, B, it, bseed, true, lr, tp.beta1, tp.beta2, tp.adam_eps, (float)(1.0 / (1.0 - p1)),
+                                  (float)(1.0 / (1.0 - p2)), s->stream);
+        if (e != hipSuccess) return fail(s, NB_E_HIP, "training launch: %s", hipGetErrorString(e));
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    const auto t2 = std::chrono::steady_clock::now();
+
+    std::vector<float> blob((size_t)NB_MODEL_FLOATS), st, lh;
+    r = fetch(s, P, (const float *)t.P, (size_t)nb::kMlpParams);
+    if (!r) r = fetch(s, st, (const float *)stats, (size_t)2 * (nb::kFeat + nb::kOut));
+    if (!r && tp.iters > 0) r = fetch(s, lh, (const float *)t.loss_hist, (size_t)tp.iters);
+    if (r) return r;
+    std::copy(P.begin(), P.end(), blob.begin());
+    std::copy(st.begin(), st.end(), blob.begin() + nb::kMlpParams);
+    restore.v = s->p.mode != NB_MODE_PLAIN ? 1 : 0;
+    r = install_model(s, blob.data());
+    if (r) return r;
+    if (rep) {
+        std::memset(rep, 0, sizeof *rep);
+        rep->samples = S;
+        rep->loss_first = lh.empty() ? 0.0f : lh.front();
+        rep->loss_last = lh.empty() ? 0.0f : lh.back();
+        rep->gen_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        rep->train_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    }
+    return NB_OK;
+}
+
+int nb_coarse_features(nb_sim *s, float *feat, float *acc3, float *e2)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    if (s->p.vote_width != 64) return fail(s, NB_E_ARG, "the coarse traversal needs vote width 64");
+    HIP_TRY(s, hipSetDevice(s->device));
+    nb::Dev &d = s->d;
+    int r = ensure_quad(s);
+    if (r) return r;
+    const int saved_quad = d.quad;
+    d.quad = 1;
+    struct QuadRestore {
+        nb::Dev &d;
+        int v;
+        ~QuadRestore() { d.quad = v; }
+    } restore{d, saved_quad};
+    const size_t n = (size_t)d.n;
+    Scratch m(s);
+    float *xr = nullptr;
+    float4 *fr = nullptr;
+    r = m.get(&xr, n * nb::kFeat);
+    if (!r) r = m.get(&fr, 2 * n);
+    if (r) return r;
+    nb::Ctl c0;
+    r = read_ctl(s, &c0);
+    if (r) return r;
+    for (int st = NB_STAGE_BBOX; st <= NB_STAGE_SORT; ++st) {
+        r = launch_stage(s, st);
+        if (r) return r;
+    }
+    hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, dq_coarse_ptr(s), nullptr, 0, xr, fr, nullptr, 0, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "coarse launch: %s", hipGetErrorString(e));
+    HIP_TRY(s, hipMemcpyAsync(&d.ctl->step, &c0.step, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    r = check_device_error(s);
+    if (r) return r;
+    std::vector<float> hx;
+    std::vector<float4> hf;
+    std::vector<int> srt;
+    r = fetch(s, hx, (const float *)xr, n * nb::kFeat);
+    if (!r) r = fetch(s, hf, (const float4 *)fr, 2 * n);
+    if (!r) r = fetch(s, srt, (const int *)d.sorted, n);
+    if (r) return r;
+    for (size_t pos = 0; pos < n; ++pos) {
+        const size_t b = (size_t)srt[pos];
+        if (feat) std::memcpy(feat + b * nb::kFeat, hx.data() + pos * nb::kFeat, nb::kFeat * sizeof(float));
+        const float4 a = hf[2 * pos], w = hf[2 * pos + 1];
+        if (acc3) {
+            acc3[3 * b + 0] = a.x;
+            acc3[3 * b + 1] = a.y;
+            acc3[3 * b + 2] = a.z;
+        }
+        if (e2) {
+            const float inv = a.w > 0.f ? 1.0f / a.w : 0.f;
+            e2[3 * b + 0] = w.x * inv;
+            e2[3 * b + 1] = w.y * inv;
+            e2[3 * b + 2] = w.z * inv;
+        }
+    }
+    return NB_OK;
+}
+
+int nb_mlp_forward(nb_sim *s, const float *x, int32_t n, float *y)
+{
+    if (!s || n < 0 || (n > 0 && (!x || !y))) return NB_E_ARG;
+    if (!s->have_model) return fail(s, NB_E_MODEL, "no model installed");
+    if (n == 0) return NB_OK;
+    HIP_TRY(s, hipSetDevice(s->device));
+    Scratch m(s);
+    float *xd = nullptr, *yd = nullptr;
+    uint16_t *fb = nullptr;
+    int r = m.get(&xd, (size_t)n * nb::kFeat);
+    if (!r) r = m.get(&fb, (size_t)n * nb::kFeat);
+    if (!r) r = m.get(&yd, (size_t)n * nb::kOut);
+    if (r) return r;
+    HIP_TRY(s, hipMemcpyAsync(xd, x, (size_t)n * nb::kFeat * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    hipError_t e = nb::launch_norm_bf16(s->L, xd, n, fb, s->stream);
+    if (e == hipSuccess) e = nb::launch_mlp(s->d, s->L, fb, n, 1, yd, 0, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "network launch: %s", hipGetErrorString(e));
+    HIP_TRY(s, hipMemcpyAsync(y, yd, (size_t)n * nb::kOut * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+namespace {
+// Full-batch passes of the FP32 training kernels on host data, from the
+// installed model's parameters (the test path of the training kernels).
+int host_data_passes(nb_sim *s, const float *x, const float *t, int n, int iters, bool adam, float lr, float *loss,
+                     float *out)
+{
+    if (!s || !x || !t || n < 1 || iters < 1) return NB_E_ARG;
+    if (!s->have_model) return fail(s, NB_E_MODEL, "no model installed");
+    HIP_TRY(s, hipSetDevice(s->device));
+    Scratch m(s);
+    nb::Train tr{};
+    float *xd = nullptr, *td = nullptr;
+    int r = train_buffers(m, tr, n, n, iters);
+    if (!r) r = m.get(&xd, (size_t)n * nb::kFeat);
+    if (!r) r = m.get(&td, (size_t)n * nb::kOut);
+    if (r) return r;
+    tr.X = xd;
+    tr.T = td;
+    HIP_TRY(s, hipMemcpyAsync(xd, x, (size_t)n * nb::kFeat * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(td, t, (size_t)n * nb::kOut * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(tr.P, s->model.data(), (size_t)nb::kMlpParams * sizeof(float), hipMemcpyHostToDevice,
+                              s->stream));
+    HIP_TRY(s, hipMemsetAsync(tr.M, 0, (size_t)nb::kMlpParams * sizeof(float), s->stream));
+    HIP_TRY(s, hipMemsetAsync(tr.V, 0, (size_t)nb::kMlpParams * sizeof(float), s->stream));
+    double p1 = 1.0, p2 = 1.0;
+    for (int it = 0; it < iters; ++it) {
+        p1 *= 0.9;
+        p2 *= 0.999;
+        hipError_t e = nb::launch_train_iter(tr, n, it, 0, adam, lr, 0.9f, 0.999f, 1e-8f, (float)(1.0 / (1.0 - p1)),
+                                             (float)(1.0 / (1.0 - p2)), s->stream);
+        if (e != hipSuccess) return fail(s, NB_E_HIP, "training launch: %s", hipGetErrorString(e));
+    }
+    if (loss) HIP_TRY(s, hipMemcpyAsync(loss, tr.loss_hist, sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(out, adam ? tr.P : tr.G, (size_t)nb::kMlpParams * sizeof(float), hipMemcpyDeviceToHost,
+                              s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+} // namespace
+
+int nb_mlp_loss_grad(nb_sim *s, const float *x, const float *t, int32_t n, float *loss, float *grad)
+{
+    if (!grad) return NB_E_ARG;
+    return host_data_passes(s, x, t, n, 1, false, 0.0f, loss, grad);
+}
+
+int nb_mlp_adam(nb_sim *s, const float *x, const float *t, int32_t n, int32_t iters, float lr, float *params)
+{
+    if (!params || !(lr > 0.0f)) return NB_E_ARG;
+    return host_data_passes(s, x, t, n, iters, true, lr, nullptr, params);
 }
 
 int nb_get_counters(nb_sim *s, nb_counters *out)

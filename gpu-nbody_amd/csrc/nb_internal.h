@@ -17,9 +17,11 @@
 //                                   and mass, or the child cell's FP32 centre of
 //                                   mass and mass -- a popped cell is one
 //                                   contiguous 160-byte read in the traversal
-//            cq64  : double[6*cap]  traceless quadrupole about the centre of mass
-//                                   (xx, yy, zz, xy, xz, yz), learned mode only
-//            cqa/cqb: float4/float2[8*cap] the children's quadrupoles, next to cdata
+//            cq64  : double[8*cap]  second moments about the centre of mass, learned
+//                                   mode only: traceless Q (xx, yy, zz, xy, xz, yz),
+//                                   trace T, 0
+//            cqa/cqb: float4/float2[8*cap] each child's (Qxx, Qyy, Qxy, Qxz) and
+//                                   (Qyz, T eps^2), next to cdata
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
 //            sorted : int32[N]      body index at each sorted position
@@ -114,5 +116,64 @@ hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, 
 hipError_t launch_integrate(const Dev &d, hipStream_t st);
 hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st);
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st);
+
+// ---- learned coarse-theta correction (nb_learned.hip) -----------------------
+constexpr int kFeat = 24, kHid = 64, kOut = 3;
+constexpr int kMlpParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOut + kOut; // 5955
+// parameter offsets: W1[24][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]
+constexpr int kOffB1 = kFeat * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kOut;
+
+// Inference network for the matrix cores: BF16 weights stored per output
+// neuron (w1t[64][32], inputs 24..31 zero; w2t[64][64]; w3t[16][64], outputs
+// 3..15 zero, target scale folded in), FP32 biases (b3 has 16 entries with the
+// target mean folded in) and the feature normalization (mean, 1/sd).
+struct Mlp {
+    const uint16_t *w1t, *w2t, *w3t;
+    const float *b1, *b2, *b3;
+    const float *xmu, *xrs;
+};
+constexpr size_t kMlpBufBytes = (64 * 32 + 64 * 64 + 16 * 64) * 2 + (64 + 64 + 16 + 24 + 24) * 4;
+
+struct Learn {
+    uint16_t *feat; // bf16 [N][24] normalized features, by sorted position
+    float4 *frame;  // [2N] by sorted position: (a, |a|) and (|a| e2, 0)
+    Mlp mlp;
+};
+
+// What the coarse traversal writes: normalized BF16 features and frames for
+// every body (inference), raw FP32 features and frames for listed groups
+// (training pairs), accelerations of listed groups (training targets), or the
+// finished body update (the coarse method alone).
+enum CoarseOut { kOutInfer = 0, kOutRaw = 1, kOutAccel = 2, kOutFinish = 3 };
+hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,
+                         int ngroups, float *xraw, float4 *frame_out, float4 *acc_out, int fmode,
+                         hipStream_t st);
+// BF16 matrix-core inference over rows [0, rows) of feat: sink 0 applies the
+// correction and finishes the bodies (fmode as launch_force), sink 1 writes
+// the raw residual y[rows][3]
+hipError_t launch_mlp(const Dev &d, const Learn &L, const uint16_t *feat, int rows, int sink, float *y,
+                      int fmode, hipStream_t st);
+// raw features -> normalized BF16 (the inference input format)
+hipError_t launch_norm_bf16(const Learn &L, const float *xraw, int rows, uint16_t *out, hipStream_t st);
+// training targets (a_fine - a) in the frame over |a|
+hipError_t launch_make_target(const float4 *frame, const float4 *afine, int rows, float *t, hipStream_t st);
+// per-column mean and standard deviation (sd 1 for a constant column), then
+// x = (x - mean) / sd in place
+hipError_t launch_standardize(float *x, int rows, int cols, float *mu, float *sd, hipStream_t st);
+hipError_t launch_gather_acc(const Dev &d, const int *idx, int ns, float4 *out, hipStream_t st);
+
+struct Train {
+    float *P, *G, *M, *V;  // parameters, gradient, Adam moments (kMlpParams each)
+    const float *X, *T;    // dataset: normalized features [S][24], targets [S][3]
+    int S;
+    float *Z1, *H1, *Z2, *H2, *dZ1, *dZ2, *dY; // [B][64] (dY [B][3])
+    float *loss_part;      // one per 256 rows of the batch
+    float *loss_hist;      // one per iteration
+};
+// forward + backward of one minibatch (rows drawn with `seed`, or rows 0..B-1
+// when seed is 0) into t.G and t.loss_hist[it]; then, if adam, one Adam step
+hipError_t launch_train_iter(const Train &t, int B, int it, uint64_t seed, bool adam, float lr, float beta1,
+                             float beta2, float eps, float c1, float c2, hipStream_t st);
 
 } // namespace nb

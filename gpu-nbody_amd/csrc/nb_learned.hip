@@ -1,0 +1,813 @@
+// nb_learned.hip -- the learned coarse-theta correction for gfx950.
+//
+// Coarse traversal. The wave-uniform traversal of the force stage (one wave
+// = one vote group of 64 sorted bodies) at a coarser opening angle. Every
+// accepted cell contributes its monopole and its quadrupole under the
+// softened kernel: with d = centre of mass - body, s^2 = |d|^2 + eps^2 and the
+// cell's traceless second moment Q and trace T about its centre of mass,
+//     a = m d / s^3 - Q d / s^5 + 5/2 (d.Q.d - T eps^2) d / s^7 .
+// (The T eps^2 term is the part of the second-order expansion of the
+// softened kernel that the traceless Q does not carry; it matters once cells
+// sit within a few softening lengths.) Bodies interact as point masses.
+//
+// Features. Per body, 24 numbers in a frame built from the body's own coarse
+// acceleration a: e1 = a / |a|, e2 = the part of the quadrupole acceleration
+// a_q perpendicular to e1, normalized, e3 = e1 x e2. Vectors are projected on
+// the frame and divided by |a|, so the features do not change under rotations
+// and under a common scaling of the masses.
+//    0-2  acceleration from bodies (a_b)
+//    3-4  a_q along e1, e2 (its e3 component is 0 by construction)
+//    5-7  sum over accepted cells of u * (cell monopole), u = dq[depth] / s^2
+//         in (0, 1]: how close the cell came to being opened
+//    8-10 the same weighted by u^2
+//   11-12 log(1 + sum u), log(1 + sum u^2)
+//   13-14 log(1 + accepted cells), log(1 + bodies met)
+//   15-16 log(|a_q| / |a|), log(|a_b| / |a|)
+//   17    mass-weighted mean u of the accepted cells
+//   18    accepted-cell mass / (accepted-cell mass + body mass met)
+//   19-21 offset of the body from its group's centroid in the frame, over the
+//         group radius
+//   22    log(group radius / root radius)
+//   23    cosine between the cell monopole part and a_b
+//
+// Network. 24 -> 64 -> 64 -> 3 with tanh-GELU. Inference: one wave runs its
+// 64 bodies through v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
+// accumulation): 16 + 32 + 8 matrix instructions per wave; activations pass
+// between layers through LDS. Training: FP32 kernels for the forward pass,
+// the backward pass and Adam, with deterministic reductions.
+#include "nb_device.h"
+
+namespace nb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// numerics shared by inference and training
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    // round to nearest even (inputs are finite)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// tanh(y) = 1 - 2 / (exp(2y) + 1); saturates correctly at both ends
+__device__ __forceinline__ float tanh_fast(float y)
+{
+    return 1.0f - 2.0f / (__expf(2.0f * y) + 1.0f);
+}
+
+constexpr float kGeluC = 0.7978845608028654f; // sqrt(2 / pi)
+constexpr float kGeluA = 0.044715f;
+
+__device__ __forceinline__ float gelu(float x)
+{
+    const float t = tanh_fast(kGeluC * (x + kGeluA * x * x * x));
+    return 0.5f * x * (1.0f + t);
+}
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float x2 = x * x;
+    const float t = tanh_fast(kGeluC * (x + kGeluA * x2 * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x2);
+}
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t *p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(p));
+}
+
+// ---------------------------------------------------------------------------
+// Coarse traversal
+// ---------------------------------------------------------------------------
+template <int OUT>
+__global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
+                                                  const float4 *__restrict__ cdata,
+                                                  const float4 *__restrict__ cqa,
+                                                  const float2 *__restrict__ cqb,
+                                                  const float *__restrict__ dqtab,
+                                                  const int *__restrict__ groups, int ngroups,
+                                                  float *__restrict__ xraw, float4 *__restrict__ frame_out,
+                                                  float4 *__restrict__ acc_out, int fmode)
+{
+    constexpr bool kFeatOn = OUT == kOutInfer || OUT == kOutRaw;
+    __shared__ unsigned stk[4][kStack];
+    __shared__ float dq_s[kMaxDepth];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int n = d.n;
+    Ctl *ctl = d.ctl;
+    if (ctl->error) return;
+    if (ctl->max_depth > kMaxDepth) {
+        if (tid == 0) flag_error(ctl, kErrDepth);
+        return;
+    }
+    if (tid < kMaxDepth) dq_s[tid] = dqtab[tid];
+    const int step = ctl->step;
+    const float root_r = ctl->radius;
+    __syncthreads();
+
+    const int gslot = blockIdx.x * 4 + wv;
+    int g = gslot;
+    if (groups) {
+        if (gslot >= ngroups) return;
+        g = groups[gslot];
+    }
+    const int pos = g * 64 + lane;
+    const bool valid = pos >= 0 && pos < n;
+    if (__ballot(valid) == 0ull) return; // wave-uniform
+    const float4 me = valid ? d.bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float eps2 = d.eps2;
+    const int cap = d.cap;
+
+    if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
+    int sp = 1;
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f; // total, Kahan
+    float bx = 0.f, by = 0.f, bz = 0.f;                                // bodies
+    float qx = 0.f, qy = 0.f, qz = 0.f;                                // quadrupole part
+    float ux = 0.f, uy = 0.f, uz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;  // u, u^2 weighted monopoles
+    float su = 0.f, sv = 0.f, smu = 0.f, smc = 0.f, smb = 0.f;
+    int nc = 0, nbod = 0;
+    long long ninter = 0;
+    bool bad = false;
+    while (sp > 0) {
+        __builtin_amdgcn_wave_barrier();
+        --sp;
+        const unsigned e = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
+        const int c = (int)(e >> 6);
+        const int dep = (int)(e & 63u);
+        const float thr = dq_s[dep];
+        const int *chp = cchild + (size_t)c * 8;
+        const float4 *cdp = cdata + (size_t)c * 8;
+        int ch[8];
+        float4 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ch[k] = __builtin_amdgcn_readfirstlane(chp[k]);
+            q[k] = cdp[k];
+        }
+        float pbx = 0.f, pby = 0.f, pbz = 0.f, pcx = 0.f, pcy = 0.f, pcz = 0.f;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = ch[k];
+            if (v < 0) break;
+            const float dx = q[k].x - me.x, dy = q[k].y - me.y, dz = q[k].z - me.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            if (v < n) {
+                const float ri = __builtin_amdgcn_rsqf(d2);
+                const float f = q[k].w * (ri * ri * ri);
+                pbx = fmaf(dx, f, pbx);
+                pby = fmaf(dy, f, pby);
+                pbz = fmaf(dz, f, pbz);
+                ++cnt;
+                if (kFeatOn) {
+                    ++nbod;
+                    smb += q[k].w;
+                }
+            } else if (__ballot(!valid || d2 >= thr) == __ballot(1)) {
+                const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qxy, Qxz)
+                const float2 qb = cqb[(size_t)c * 8 + k]; // (Qyz, T eps^2)
+                const float ri = __builtin_amdgcn_rsqf(d2);
+                const float r2i = ri * ri;
+                const float r3 = r2i * ri;
+                const float f = q[k].w * r3;
+                const float qzz = -(qa.x + qa.y);
+                const float Qx = fmaf(qa.x, dx, fmaf(qa.z, dy, qa.w * dz));
+                const float Qy = fmaf(qa.z, dx, fmaf(qa.y, dy, qb.x * dz));
+                const float Qz = fmaf(qa.w, dx, fmaf(qb.x, dy, qzz * dz));
+                const float dQd = fmaf(Qx, dx, fmaf(Qy, dy, Qz * dz));
+                const float r5 = r3 * r2i;
+                const float c2 = 2.5f * (dQd - qb.y) * (r5 * r2i);
+                const float gx = fmaf(c2, dx, -r5 * Qx);
+                const float gy = fmaf(c2, dy, -r5 * Qy);
+                const float gz = fmaf(c2, dz, -r5 * Qz);
+                pcx += fmaf(dx, f, gx);
+                pcy += fmaf(dy, f, gy);
+                pcz += fmaf(dz, f, gz);
+                ++cnt;
+                if (kFeatOn) {
+                    qx += gx;
+                    qy += gy;
+                    qz += gz;
+                    const float u = thr * r2i;
+                    const float fu = f * u;
+                    ux = fmaf(fu, dx, ux);
+                    uy = fmaf(fu, dy, uy);
+                    uz = fmaf(fu, dz, uz);
+                    const float fu2 = fu * u;
+                    vx = fmaf(fu2, dx, vx);
+                    vy = fmaf(fu2, dy, vy);
+                    vz = fmaf(fu2, dz, vz);
+                    su += u;
+                    sv = fmaf(u, u, sv);
+                    smu = fmaf(u, q[k].w, smu);
+                    smc += q[k].w;
+                    ++nc;
+                }
+            } else {
+                // the tree is at most kMaxDepth deep: anything else is corrupt
+                if (sp >= kStack || v - n >= cap || dep + 1 >= kMaxDepth) {
+                    bad = true;
+                    break;
+                }
+                if (lane == 0) stk[wv][sp] = ((unsigned)(v - n) << 6) | (unsigned)(dep + 1);
+                ++sp;
+            }
+        }
+        kahan_add(sx, kx, pbx + pcx);
+        kahan_add(sy, ky, pby + pcy);
+        kahan_add(sz, kz, pbz + pcz);
+        if (kFeatOn) {
+            bx += pbx;
+            by += pby;
+            bz += pbz;
+        }
+        ninter += cnt;
+        if (bad) break;
+    }
+    if (bad && lane == 0) flag_error(ctl, kErrStack);
+    const float ax = sx - kx, ay = sy - ky, az = sz - kz;
+    const int slot = gslot * 64 + lane;
+    if constexpr (OUT == kOutAccel) {
+        if (valid) acc_out[slot] = make_float4(ax, ay, az, 0.f);
+    } else if constexpr (OUT == kOutFinish) {
+        if (valid) finish_body(d, pos, me, ax, ay, az, fmode, step);
+    } else {
+        // group centroid and radius (wave reductions over the valid lanes)
+        const float nv = (float)__popcll(__ballot(valid));
+        float gx = valid ? me.x : 0.f, gy = valid ? me.y : 0.f, gz = valid ? me.z : 0.f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            gx += __shfl_xor(gx, o);
+            gy += __shfl_xor(gy, o);
+            gz += __shfl_xor(gz, o);
+        }
+        gx /= nv;
+        gy /= nv;
+        gz /= nv;
+        const float ox = me.x - gx, oy = me.y - gy, oz = me.z - gz;
+        float rg = valid ? sqrtf(ox * ox + oy * oy + oz * oz) : 0.f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) rg = fmaxf(rg, __shfl_xor(rg, o));
+
+        float f[kFeat];
+#pragma unroll
+        for (int i = 0; i < kFeat; ++i) f[i] = 0.f;
+        const float A = sqrtf(ax * ax + ay * ay + az * az);
+        float e2x = 0.f, e2y = 0.f, e2z = 0.f;
+        if (A > 0.f) {
+            const float inv = 1.0f / A;
+            const float e1x = ax * inv, e1y = ay * inv, e1z = az * inv;
+            const float qe1 = qx * e1x + qy * e1y + qz * e1z;
+            float wx = qx - qe1 * e1x, wy = qy - qe1 * e1y, wz = qz - qe1 * e1z;
+            float W = sqrtf(wx * wx + wy * wy + wz * wz);
+            if (!(W > 0.f)) {
+                // no quadrupole direction: the coordinate axis least aligned with e1
+                const float fx = fabsf(e1x), fy = fabsf(e1y), fz = fabsf(e1z);
+                const float tx = (fx <= fy && fx <= fz) ? 1.f : 0.f;
+                const float ty = (tx == 0.f && fy <= fz) ? 1.f : 0.f;
+                const float tz = (tx == 0.f && ty == 0.f) ? 1.f : 0.f;
+                const float te1 = tx * e1x + ty * e1y + tz * e1z;
+                wx = tx - te1 * e1x;
+                wy = ty - te1 * e1y;
+                wz = tz - te1 * e1z;
+                W = sqrtf(wx * wx + wy * wy + wz * wz);
+            }
+            e2x = wx / W;
+            e2y = wy / W;
+            e2z = wz / W;
+            const float e3x = e1y * e2z - e1z * e2y;
+            const float e3y = e1z * e2x - e1x * e2z;
+            const float e3z = e1x * e2y - e1y * e2x;
+#define NB_LOC(X, Y, Z, I)                                                                         \
+    f[I] = ((X) * e1x + (Y) * e1y + (Z) * e1z) * inv;                                             \
+    f[(I) + 1] = ((X) * e2x + (Y) * e2y + (Z) * e2z) * inv;                                       \
+    f[(I) + 2] = ((X) * e3x + (Y) * e3y + (Z) * e3z) * inv
+            NB_LOC(bx, by, bz, 0);
+            f[3] = qe1 * inv;
+            f[4] = (qx * e2x + qy * e2y + qz * e2z) * inv;
+            NB_LOC(ux, uy, uz, 5);
+            NB_LOC(vx, vy, vz, 8);
+            f[11] = log1pf(su);
+            f[12] = log1pf(sv);
+            f[13] = log1pf((float)nc);
+            f[14] = log1pf((float)nbod);
+            const float qn = sqrtf(qx * qx + qy * qy + qz * qz);
+            const float bn = sqrtf(bx * bx + by * by + bz * bz);
+            f[15] = logf(fmaxf(qn * inv, 1e-30f));
+            f[16] = logf(fmaxf(bn * inv, 1e-30f));
+            f[17] = smc > 0.f ? smu / smc : 0.f;
+            f[18] = (smc + smb) > 0.f ? smc / (smc + smb) : 0.f;
+            if (rg > 0.f) {
+                const float ir = 1.0f / rg;
+                f[19] = (ox * e1x + oy * e1y + oz * e1z) * ir;
+                f[20] = (ox * e2x + oy * e2y + oz * e2z) * ir;
+                f[21] = (ox * e3x + oy * e3y + oz * e3z) * ir;
+            }
+            f[22] = logf(fmaxf(rg / root_r, 1e-30f));
+            const float mx = ax - bx - qx, my = ay - by - qy, mz = az - bz - qz;
+            const float mn = sqrtf(mx * mx + my * my + mz * mz);
+            f[23] = (mn > 0.f && bn > 0.f) ? (mx * bx + my * by + mz * bz) / (mn * bn) : 0.f;
+#undef NB_LOC
+        }
+        if (valid) {
+            const float4 fr0 = make_float4(ax, ay, az, A);
+            const float4 fr1 = make_float4(A * e2x, A * e2y, A * e2z, 0.f);
+            if constexpr (OUT == kOutRaw) {
+                float *o = xraw + (size_t)slot * kFeat;
+#pragma unroll
+                for (int i = 0; i < kFeat; i += 4)
+                    *reinterpret_cast<float4 *>(o + i) = make_float4(f[i], f[i + 1], f[i + 2], f[i + 3]);
+                frame_out[2 * (size_t)slot] = fr0;
+                frame_out[2 * (size_t)slot + 1] = fr1;
+            } else {
+                uint32_t w[kFeat / 2];
+#pragma unroll
+                for (int i = 0; i < kFeat; i += 2) {
+                    const uint32_t lo = to_bf16((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]);
+                    const uint32_t hi = to_bf16((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]);
+                    w[i / 2] = lo | (hi << 16);
+                }
+                uint4 *o = reinterpret_cast<uint4 *>(L.feat + (size_t)pos * kFeat);
+                o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+                o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+                o[2] = make_uint4(w[8], w[9], w[10], w[11]);
+                L.frame[2 * (size_t)pos] = fr0;
+                L.frame[2 * (size_t)pos + 1] = fr1;
+            }
+        }
+    }
+    // device interaction counter: one atomic per wave
+    const unsigned long long nvalid = (unsigned long long)__popcll(__ballot(valid));
+    if (lane == 0 && ninter && nvalid)
+        __hip_atomic_fetch_add(&ctl->interactions, (unsigned long long)ninter * nvalid, __ATOMIC_RELAXED,
+                               AGENT);
+}
+
+hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,
+                         int ngroups, float *xraw, float4 *frame_out, float4 *acc_out, int fmode,
+                         hipStream_t st)
+{
+    const int waves = groups ? ngroups : (d.n + 63) / 64;
+    if (waves <= 0) return hipSuccess;
+    const dim3 grid((waves + 3) / 4), block(256);
+#define NB_COARSE_ARGS                                                                             \
+    d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float2 *)d.cqb, \
+        dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
+    switch (out) {
+    case kOutInfer: hipLaunchKernelGGL(k_coarse64<kOutInfer>, grid, block, 0, st, NB_COARSE_ARGS); break;
+    case kOutRaw: hipLaunchKernelGGL(k_coarse64<kOutRaw>, grid, block, 0, st, NB_COARSE_ARGS); break;
+    case kOutAccel: hipLaunchKernelGGL(k_coarse64<kOutAccel>, grid, block, 0, st, NB_COARSE_ARGS); break;
+    case kOutFinish: hipLaunchKernelGGL(k_coarse64<kOutFinish>, grid, block, 0, st, NB_COARSE_ARGS); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef NB_COARSE_ARGS
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// BF16 matrix-core inference
+//
+// v_mfma_f32_16x16x32_bf16 operand layout (lane l, i16 = l % 16, grp = l / 16):
+//   A: row i16 of the 16-row block, k = 8 grp .. 8 grp + 7
+//   B: column i16 of the 16-column block, k = 8 grp .. 8 grp + 7
+//   D: column i16, rows 4 grp .. 4 grp + 3
+// Weights are stored per output neuron (w_t[n][k]) so that a B operand is one
+// 16-byte load; the activations of a wave's 64 bodies go through LDS as
+// row-major BF16 with a padded row stride.
+// ---------------------------------------------------------------------------
+constexpr int kHS = 72; // LDS row stride of the activations, in bf16 (144 bytes)
+
+template <int SINK>
+__global__ __launch_bounds__(256) void k_mlp(Dev d, Learn L, const uint16_t *__restrict__ feat, int rows,
+                                             float *__restrict__ yout, int fmode)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t hs[4][64 * kHS];
+    __shared__ float ys[4][64 * 4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int base = (blockIdx.x * 4 + wv) * 64;
+    const int i16 = lane & 15, grp = lane >> 4;
+    uint16_t *h = hs[wv];
+    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+    // layer 1: [64 x 32] x [32 x 64]
+    bf16x8 a[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const int row = base + rb * 16 + i16;
+        if (grp < 3 && row < rows)
+            a[rb] = load_bf16x8(feat + (size_t)row * kFeat + grp * 8);
+        else
+            a[rb] = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const bf16x8 b = load_bf16x8(L.mlp.w1t + (cb * 16 + i16) * 32 + grp * 8);
+        const float bias = L.mlp.b1[cb * 16 + i16];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb], b, zero4, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[(rb * 16 + grp * 4 + j) * kHS + cb * 16 + i16] = to_bf16(gelu(z[j] + bias));
+        }
+    }
+    __syncthreads();
+
+    // layer 2: [64 x 64] x [64 x 64]; all A operands are read before H2 overwrites H1
+    bf16x8 a2[2][4];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) a2[kb][rb] = load_bf16x8(h + (rb * 16 + i16) * kHS + kb * 32 + grp * 8);
+    __syncthreads();
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const bf16x8 b0 = load_bf16x8(L.mlp.w2t + (cb * 16 + i16) * 64 + grp * 8);
+        const bf16x8 b1 = load_bf16x8(L.mlp.w2t + (cb * 16 + i16) * 64 + 32 + grp * 8);
+        const float bias = L.mlp.b2[cb * 16 + i16];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][rb], b0, zero4, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][rb], b1, z, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[(rb * 16 + grp * 4 + j) * kHS + cb * 16 + i16] = to_bf16(gelu(z[j] + bias));
+        }
+    }
+    __syncthreads();
+
+    // layer 3: [64 x 64] x [64 x 16] (outputs 3..15 are zero columns)
+    {
+        const bf16x8 b0 = load_bf16x8(L.mlp.w3t + i16 * 64 + grp * 8);
+        const bf16x8 b1 = load_bf16x8(L.mlp.w3t + i16 * 64 + 32 + grp * 8);
+        const float bias = L.mlp.b3[i16];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const bf16x8 x0 = load_bf16x8(h + (rb * 16 + i16) * kHS + grp * 8);
+            const bf16x8 x1 = load_bf16x8(h + (rb * 16 + i16) * kHS + 32 + grp * 8);
+            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0, b0, zero4, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1, b1, z, 0, 0, 0);
+            if (i16 < 4) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) ys[wv][(rb * 16 + grp * 4 + j) * 4 + i16] = z[j] + bias;
+            }
+        }
+    }
+    __syncthreads();
+
+    const int row = base + lane;
+    if (row >= rows) return;
+    const float y0 = ys[wv][lane * 4 + 0], y1 = ys[wv][lane * 4 + 1], y2 = ys[wv][lane * 4 + 2];
+    if constexpr (SINK == 1) {
+        yout[(size_t)row * 3 + 0] = y0;
+        yout[(size_t)row * 3 + 1] = y1;
+        yout[(size_t)row * 3 + 2] = y2;
+    } else {
+        // residual = |a| (y0 e1 + y1 e2 + y2 e3) = y0 a + y1 (|a| e2) + y2 (a x |a| e2) / |a|
+        const float4 f0 = L.frame[2 * (size_t)row];
+        const float4 f1 = L.frame[2 * (size_t)row + 1];
+        float ax = f0.x, ay = f0.y, az = f0.z;
+        if (f0.w > 0.f) {
+            const float inv = 1.0f / f0.w;
+            const float cx = (f0.y * f1.z - f0.z * f1.y) * inv;
+            const float cy = (f0.z * f1.x - f0.x * f1.z) * inv;
+            const float cz = (f0.x * f1.y - f0.y * f1.x) * inv;
+            ax += y0 * f0.x + y1 * f1.x + y2 * cx;
+            ay += y0 * f0.y + y1 * f1.y + y2 * cy;
+            az += y0 * f0.z + y1 * f1.z + y2 * cz;
+        }
+        finish_body(d, row, d.bnode[row], ax, ay, az, fmode, d.ctl->step);
+    }
+}
+
+hipError_t launch_mlp(const Dev &d, const Learn &L, const uint16_t *feat, int rows, int sink, float *y,
+                      int fmode, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    const dim3 grid((rows + 255) / 256), block(256);
+    if (sink == 0)
+        hipLaunchKernelGGL(k_mlp<0>, grid, block, 0, st, d, L, feat, rows, y, fmode);
+    else
+        hipLaunchKernelGGL(k_mlp<1>, grid, block, 0, st, d, L, feat, rows, y, fmode);
+    return hipGetLastError();
+}
+
+__global__ void k_norm_bf16(Mlp m, const float *__restrict__ x, int rows, uint16_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * kFeat) return;
+    const int c = i % kFeat;
+    out[i] = to_bf16((x[i] - m.xmu[c]) * m.xrs[c]);
+}
+
+hipError_t launch_norm_bf16(const Learn &L, const float *xraw, int rows, uint16_t *out, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    const int total = rows * kFeat;
+    hipLaunchKernelGGL(k_norm_bf16, dim3((total + 255) / 256), dim3(256), 0, st, L.mlp, xraw, rows, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Training pairs
+// ---------------------------------------------------------------------------
+__global__ void k_make_target(const float4 *__restrict__ frame, const float4 *__restrict__ afine, int rows,
+                              float *__restrict__ t)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const float4 f0 = frame[2 * (size_t)i], f1 = frame[2 * (size_t)i + 1];
+    const float4 af = afine[i];
+    float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+    if (f0.w > 0.f) {
+        const float inv = 1.0f / f0.w;
+        const float e1x = f0.x * inv, e1y = f0.y * inv, e1z = f0.z * inv;
+        const float e2x = f1.x * inv, e2y = f1.y * inv, e2z = f1.z * inv;
+        const float e3x = e1y * e2z - e1z * e2y, e3y = e1z * e2x - e1x * e2z, e3z = e1x * e2y - e1y * e2x;
+        const float rx = af.x - f0.x, ry = af.y - f0.y, rz = af.z - f0.z;
+        t0 = (rx * e1x + ry * e1y + rz * e1z) * inv;
+        t1 = (rx * e2x + ry * e2y + rz * e2z) * inv;
+        t2 = (rx * e3x + ry * e3y + rz * e3z) * inv;
+    }
+    t[3 * (size_t)i + 0] = t0;
+    t[3 * (size_t)i + 1] = t1;
+    t[3 * (size_t)i + 2] = t2;
+}
+
+hipError_t launch_make_target(const float4 *frame, const float4 *afine, int rows, float *t, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_make_target, dim3((rows + 255) / 256), dim3(256), 0, st, frame, afine, rows, t);
+    return hipGetLastError();
+}
+
+// one block per column; FP64 sums in a fixed order
+__global__ __launch_bounds__(256) void k_colstats(const float *__restrict__ x, int rows, int cols,
+                                                  float *__restrict__ mu, float *__restrict__ sd)
+{
+    __shared__ double s1[256], s2[256];
+    const int c = blockIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int r = threadIdx.x; r < rows; r += 256) {
+        const double v = (double)x[(size_t)r * cols + c];
+        a += v;
+        b += v * v;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = s1[0] / rows;
+        const double var = fmax(s2[0] / rows - m * m, 0.0);
+        const double s = sqrt(var);
+        mu[c] = (float)m;
+        sd[c] = s > 1e-12 ? (float)s : 1.0f;
+    }
+}
+
+__global__ void k_apply_standardize(float *__restrict__ x, int rows, int cols, const float *__restrict__ mu,
+                                    const float *__restrict__ sd)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    const int c = (int)(i % cols);
+    x[i] = (x[i] - mu[c]) / sd[c];
+}
+
+hipError_t launch_standardize(float *x, int rows, int cols, float *mu, float *sd, hipStream_t st)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_colstats, dim3(cols), dim3(256), 0, st, x, rows, cols, mu, sd);
+    const size_t total = (size_t)rows * cols;
+    hipLaunchKernelGGL(k_apply_standardize, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, x, rows,
+                       cols, (const float *)mu, (const float *)sd);
+    return hipGetLastError();
+}
+
+__global__ void k_gather_acc(const float4 *__restrict__ acc, const int *__restrict__ idx, int ns,
+                             float4 *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) out[i] = acc[idx[i]];
+}
+
+hipError_t launch_gather_acc(const Dev &d, const int *idx, int ns, float4 *out, hipStream_t st)
+{
+    if (ns <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_acc, dim3((ns + 255) / 256), dim3(256), 0, st, (const float4 *)d.acc4, idx, ns,
+                       out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP32 training: forward, backward, Adam
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int batch_row(uint64_t seed, int it, int r, int S)
+{
+    if (seed == 0) return r; // the whole dataset in order
+    uint64_t z = seed + 0x9e3779b97f4a7c15ull * (((uint64_t)it << 32) + (uint64_t)r + 1ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (int)(z % (uint64_t)S);
+}
+
+constexpr int kRowsPerBlock = 16;
+
+// Z = X W + b, H = gelu(Z) for 16 rows per block; K inputs (24 or 64), 64 outputs.
+// gather: X rows come from the dataset through batch_row.
+template <int K, bool GATHER>
+__global__ __launch_bounds__(256) void k_fwd_hidden(const float *__restrict__ X, uint64_t seed, int it, int S,
+                                                    const float *__restrict__ W, const float *__restrict__ b,
+                                                    int B, float *__restrict__ Z, float *__restrict__ H)
+{
+    __shared__ float w[K * 64];
+    __shared__ float xin[kRowsPerBlock][K];
+    const int t = threadIdx.x;
+    for (int i = t; i < K * 64; i += 256) w[i] = W[i];
+    const int r0 = blockIdx.x * kRowsPerBlock;
+    for (int i = t; i < kRowsPerBlock * K; i += 256) {
+        const int rl = i / K, k = i % K;
+        const int r = r0 + rl;
+        float v = 0.f;
+        if (r < B) v = GATHER ? X[(size_t)batch_row(seed, it, r, S) * K + k] : X[(size_t)r * K + k];
+        xin[rl][k] = v;
+    }
+    __syncthreads();
+    const int j = t & 63;
+    const float bj = b[j];
+    for (int rl = t >> 6; rl < kRowsPerBlock; rl += 4) {
+        const int r = r0 + rl;
+        if (r >= B) break;
+        float z = bj;
+#pragma unroll 8
+        for (int k = 0; k < K; ++k) z = fmaf(xin[rl][k], w[k * 64 + j], z);
+        Z[(size_t)r * 64 + j] = z;
+        H[(size_t)r * 64 + j] = gelu(z);
+    }
+}
+
+// Y = H2 W3 + b3; dY = 2 (Y - T) / (3 B); per-block sum of (Y - T)^2
+__global__ __launch_bounds__(256) void k_fwd_out(const float *__restrict__ H2, const float *__restrict__ W3,
+                                                 const float *__restrict__ b3, const float *__restrict__ T,
+                                                 uint64_t seed, int it, int S, int B, float *__restrict__ dY,
+                                                 float *__restrict__ loss_part)
+{
+    __shared__ float w[64 * 3];
+    __shared__ float red[256];
+    const int t = threadIdx.x;
+    if (t < 64 * 3) w[t] = W3[t];
+    __syncthreads();
+    const int r = blockIdx.x * 256 + t;
+    float l = 0.f;
+    if (r < B) {
+        float y0 = b3[0], y1 = b3[1], y2 = b3[2];
+        const float *h = H2 + (size_t)r * 64;
+        for (int k = 0; k < 64; ++k) {
+            const float hk = h[k];
+            y0 = fmaf(hk, w[k * 3 + 0], y0);
+            y1 = fmaf(hk, w[k * 3 + 1], y1);
+            y2 = fmaf(hk, w[k * 3 + 2], y2);
+        }
+        const float *tt = T + (size_t)batch_row(seed, it, r, S) * 3;
+        const float e0 = y0 - tt[0], e1 = y1 - tt[1], e2 = y2 - tt[2];
+        const float sc = 2.0f / (3.0f * (float)B);
+        dY[(size_t)r * 3 + 0] = sc * e0;
+        dY[(size_t)r * 3 + 1] = sc * e1;
+        dY[(size_t)r * 3 + 2] = sc * e2;
+        l = e0 * e0 + e1 * e1 + e2 * e2;
+    }
+    red[t] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// dZ = (dNext W^T) * gelu'(Z): W is [64][N] (N = 3 or 64); 16 rows per block
+template <int N>
+__global__ __launch_bounds__(256) void k_back_hidden(const float *__restrict__ dNext, const float *__restrict__ W,
+                                                     const float *__restrict__ Z, int B, float *__restrict__ dZ)
+{
+    __shared__ float wt[N][65]; // wt[j][k] = W[k][j], padded
+    __shared__ float dn[kRowsPerBlock][N];
+    const int t = threadIdx.x;
+    for (int i = t; i < 64 * N; i += 256) {
+        const int k = i / N, j = i % N;
+        wt[j][k] = W[i];
+    }
+    const int r0 = blockIdx.x * kRowsPerBlock;
+    for (int i = t; i < kRowsPerBlock * N; i += 256) {
+        const int rl = i / N, j = i % N;
+        const int r = r0 + rl;
+        dn[rl][j] = r < B ? dNext[(size_t)r * N + j] : 0.f;
+    }
+    __syncthreads();
+    const int k = t & 63;
+    for (int rl = t >> 6; rl < kRowsPerBlock; rl += 4) {
+        const int r = r0 + rl;
+        if (r >= B) break;
+        float s = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < N; ++j) s = fmaf(dn[rl][j], wt[j][k], s);
+        dZ[(size_t)r * 64 + k] = s * gelu_grad(Z[(size_t)r * 64 + k]);
+    }
+}
+
+// gW[k][j] = sum_r A[r][k] dZ[r][j] (block k < K), gb[j] = sum_r dZ[r][j] (block K).
+// Each thread sums every 4th row; the 4 partial sums are added in a fixed order.
+template <bool GATHER>
+__global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A, int lda, uint64_t seed, int it, int S,
+                                               const float *__restrict__ dZ, int N, int B, int K,
+                                               float *__restrict__ gW, float *__restrict__ gb)
+{
+    __shared__ float red[4][64];
+    const int t = threadIdx.x;
+    const int j = t & 63, rq = t >> 6;
+    const int k = blockIdx.x;
+    float acc = 0.f;
+    if (j < N) {
+        for (int r = rq; r < B; r += 4) {
+            float a = 1.0f;
+            if (k < K) a = GATHER ? A[(size_t)batch_row(seed, it, r, S) * lda + k] : A[(size_t)r * lda + k];
+            acc = fmaf(a, dZ[(size_t)r * N + j], acc);
+        }
+    }
+    red[rq][j] = acc;
+    __syncthreads();
+    if (rq == 0 && j < N) {
+        const float v = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+        if (k < K)
+            gW[k * N + j] = v;
+        else
+            gb[j] = v;
+    }
+}
+
+__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
+                       float *__restrict__ V, int np, float lr, float b1, float b2, float eps, float c1, float c2,
+                       const float *__restrict__ loss_part, int nparts, int B, float *__restrict__ loss_out,
+                       bool step)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        float s = 0.f;
+        for (int p = 0; p < nparts; ++p) s += loss_part[p];
+        *loss_out = s / (3.0f * (float)B);
+    }
+    if (!step || i >= np) return;
+    const float g = G[i];
+    const float m = b1 * M[i] + (1.0f - b1) * g;
+    const float v = b2 * V[i] + (1.0f - b2) * g * g;
+    M[i] = m;
+    V[i] = v;
+    P[i] -= lr * (m * c1) / (sqrtf(v * c2) + eps);
+}
+
+hipError_t launch_train_iter(const Train &t, int B, int it, uint64_t seed, bool adam, float lr, float beta1,
+                             float beta2, float eps, float c1, float c2, hipStream_t st)
+{
+    if (B <= 0) return hipErrorInvalidValue;
+    const dim3 blk(256);
+    const dim3 gh((B + kRowsPerBlock - 1) / kRowsPerBlock);
+    const int nparts = (B + 255) / 256;
+    const float *P = t.P;
+    float *G = t.G;
+    hipLaunchKernelGGL((k_fwd_hidden<kFeat, true>), gh, blk, 0, st, t.X, seed, it, t.S, P, P + kOffB1, B, t.Z1, t.H1);
+    hipLaunchKernelGGL((k_fwd_hidden<kHid, false>), gh, blk, 0, st, (const float *)t.H1, seed, it, t.S,
+                       P + kOffW2, P + kOffB2, B, t.Z2, t.H2);
+    hipLaunchKernelGGL(k_fwd_out, dim3(nparts), blk, 0, st, (const float *)t.H2, P + kOffW3, P + kOffB3, t.T, seed,
+                       it, t.S, B, t.dY, t.loss_part);
+    hipLaunchKernelGGL(k_back_hidden<kOut>, gh, blk, 0, st, (const float *)t.dY, P + kOffW3,
+                       (const float *)t.Z2, B, t.dZ2);
+    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1), blk, 0, st, (const float *)t.H2, kHid, seed, it, t.S,
+                       (const float *)t.dY, kOut, B, kHid, G + kOffW3, G + kOffB3);
+    hipLaunchKernelGGL(k_back_hidden<kHid>, gh, blk, 0, st, (const float *)t.dZ2, P + kOffW2,
+                       (const float *)t.Z1, B, t.dZ1);
+    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1), blk, 0, st, (const float *)t.H1, kHid, seed, it, t.S,
+                       (const float *)t.dZ2, kHid, B, kHid, G + kOffW2, G + kOffB2);
+    hipLaunchKernelGGL(k_wgrad<true>, dim3(kFeat + 1), blk, 0, st, t.X, kFeat, seed, it, t.S,
+                       (const float *)t.dZ1, kHid, B, kFeat, G, G + kOffB1);
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), blk, 0, st, t.P, (const float *)G, t.M, t.V,
+                       kMlpParams, lr, beta1, beta2, eps, c1, c2, (const float *)t.loss_part, nparts, B,
+                       t.loss_hist + it, adam);
+    return hipGetLastError();
+}
+
+} // namespace nb

@@ -498,9 +498,14 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     d.com64[c] = com;
     d.count[c] = cnt;
     if (d.quad) {
-        // traceless quadrupole about the centre of mass, parallel-axis rule:
-        // Q = sum_children Q_child + m_child (3 dd^T - |d|^2 I), d = child - com
-        double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // Second moments about the centre of mass for the softened quadrupole
+        // of the learned mode: the traceless tensor Q = sum m (3 s s^T - |s|^2 I)
+        // and the trace T = sum m |s|^2 (s = offset from the centre of mass),
+        // moved to the parent's centre by the parallel-axis rule. Each child's
+        // (Qxx, Qyy, Qxy, Qxz) and (Qyz, T eps^2) are also written next to its
+        // (x, y, z, m) for the traversal; bodies have no moments.
+        double q[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double eps2 = (double)d.eps2;
         for (int k = 0; k < used; ++k) {
             const int v = out[k];
             double mx, px, py, pz;
@@ -512,10 +517,10 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             } else {
                 const double4 cc = d.com64[v - n];
                 mx = cc.w; px = cc.x; py = cc.y; pz = cc.z;
-                const double *cq = d.cq64 + (size_t)(v - n) * 6;
-                for (int j = 0; j < 6; ++j) q[j] += cq[j];
-                qa = make_float4((float)cq[0], (float)cq[1], (float)cq[2], (float)cq[3]);
-                qb = make_float2((float)cq[4], (float)cq[5]);
+                const double *cq = d.cq64 + (size_t)(v - n) * 8;
+                for (int j = 0; j < 7; ++j) q[j] += cq[j];
+                qa = make_float4((float)cq[0], (float)cq[1], (float)cq[3], (float)cq[4]);
+                qb = make_float2((float)cq[5], (float)(cq[6] * eps2));
             }
             d.cqa[(size_t)c * 8 + k] = qa;
             d.cqb[(size_t)c * 8 + k] = qb;
@@ -527,13 +532,15 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             q[3] += mx * 3.0 * dx * dy;
             q[4] += mx * 3.0 * dx * dz;
             q[5] += mx * 3.0 * dy * dz;
+            q[6] += mx * r2;
         }
         for (int k = used; k < 8; ++k) {
             d.cqa[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
             d.cqb[(size_t)c * 8 + k] = make_float2(0.f, 0.f);
         }
-        double *cq = d.cq64 + (size_t)c * 6;
-        for (int j = 0; j < 6; ++j) cq[j] = q[j];
+        double *cq = d.cq64 + (size_t)c * 8;
+        for (int j = 0; j < 7; ++j) cq[j] = q[j];
+        cq[7] = 0.0;
     }
 }
 

@@ -27,6 +27,13 @@ NB_FP64_POSITIONS = 1
 NB_FUSED_UPDATE = 2
 NB_MODE_PLAIN = 0
 NB_MODE_LEARNED = 1
+NB_MODE_COARSE = 2
+
+NB_MLP_IN = 24
+NB_MLP_HIDDEN = 64
+NB_MLP_OUT = 3
+NB_MLP_PARAMS = 5955
+NB_MODEL_FLOATS = NB_MLP_PARAMS + 2 * NB_MLP_IN + 2 * NB_MLP_OUT
 
 NB_STAGE_BBOX, NB_STAGE_BUILD, NB_STAGE_SUMMARIZE, NB_STAGE_SORT, NB_STAGE_FORCE, NB_STAGE_INTEGRATE = range(1, 7)
 
@@ -62,6 +69,33 @@ class NbCounters(C.Structure):
         ("last_check_median", C.c_double),
         ("live_cells", C.c_int64),
         ("reserved", C.c_int64 * 5),
+    ]
+
+
+class NbTrainParams(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("groups_per_step", C.c_int32),
+        ("iters", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_float),
+        ("lr_final", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("adam_eps", C.c_float),
+        ("seed", C.c_uint64),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
+class NbTrainReport(C.Structure):
+    _fields_ = [
+        ("samples", C.c_int32),
+        ("loss_first", C.c_float),
+        ("loss_last", C.c_float),
+        ("gen_ms", C.c_double),
+        ("train_ms", C.c_double),
+        ("reserved", C.c_int64 * 4),
     ]
 
 
@@ -111,6 +145,15 @@ def load() -> C.CDLL:
     L.nb_probe_accel.argtypes = [vp, vp, vp, vp]
     L.nb_direct_accel.argtypes = [vp, _i32p, C.c_int32, vp]
     L.nb_force_error.argtypes = [vp, C.c_int32, C.c_uint64, vp]
+    L.nb_default_train_params.argtypes = [C.POINTER(NbTrainParams)]
+    L.nb_default_train_params.restype = None
+    L.nb_train.argtypes = [vp, C.POINTER(NbTrainParams), C.POINTER(NbTrainReport)]
+    L.nb_set_model.argtypes = [vp, _f32p, C.c_int32]
+    L.nb_get_model.argtypes = [vp, _f32p, C.c_int32]
+    L.nb_coarse_features.argtypes = [vp, vp, vp, vp]
+    L.nb_mlp_forward.argtypes = [vp, _f32p, C.c_int32, _f32p]
+    L.nb_mlp_loss_grad.argtypes = [vp, _f32p, _f32p, C.c_int32, C.POINTER(C.c_float), _f32p]
+    L.nb_mlp_adam.argtypes = [vp, _f32p, _f32p, C.c_int32, C.c_int32, C.c_float, _f32p]
     L.nb_last_error.argtypes = [vp]
     L.nb_last_error.restype = C.c_char_p
     L.nb_status_string.argtypes = [C.c_int]
@@ -118,7 +161,9 @@ def load() -> C.CDLL:
     for name in ("nb_create", "nb_set_state", "nb_step", "nb_run_stage", "nb_copy_vertices",
                  "nb_get_state", "nb_get_accel", "nb_get_positions64", "nb_get_root",
                  "nb_get_sorted", "nb_get_paths", "nb_get_cells", "nb_get_counters",
-                 "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error"):
+                 "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error",
+                 "nb_train", "nb_set_model", "nb_get_model", "nb_coarse_features",
+                 "nb_mlp_forward", "nb_mlp_loss_grad", "nb_mlp_adam"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

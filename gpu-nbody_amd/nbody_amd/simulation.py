@@ -17,7 +17,7 @@ from typing import Optional, Protocol
 import numpy as np
 
 from . import _lib
-from ._lib import NbCounters, NbError, NbParams, ptr
+from ._lib import NbCounters, NbError, NbParams, NbTrainParams, NbTrainReport, ptr
 
 
 class Mode(enum.Enum):
@@ -68,7 +68,7 @@ def _check(lib, handle, code):
 
 def make_params(nbodies: int, *, theta: Optional[float] = None, softening: Optional[float] = None,
                 dt: Optional[float] = None, vote_width: Optional[int] = None, flags: int = 0,
-                device: int = 0, pool_factor: Optional[float] = None, mode: int = _lib.NB_MODE_PLAIN,
+                device: int = 0, pool_factor: Optional[float] = None, method: int = _lib.NB_MODE_PLAIN,
                 coarse_theta: Optional[float] = None, check_tol: Optional[float] = None,
                 check_every: Optional[int] = None, check_samples: Optional[int] = None) -> NbParams:
     """nb_params with the reference's defaults (nb_default_params) and overrides."""
@@ -83,7 +83,7 @@ def make_params(nbodies: int, *, theta: Optional[float] = None, softening: Optio
             setattr(p, name, val)
     p.flags = int(flags)
     p.device = int(device)
-    p.mode = int(mode)
+    p.mode = int(method)
     return p
 
 
@@ -259,3 +259,63 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
 
     def set_timing(self, on: bool) -> None:
         _check(self._lib, self._h, self._lib.nb_set_timing(self._h, 1 if on else 0))
+
+    # -- learned coarse-theta correction -------------------------------------------
+    def train(self, **overrides) -> dict:
+        """nb_train: generate pairs over `steps` plain steps, train, install.
+        Keyword overrides of nb_train_params (steps, groups_per_step, iters,
+        batch, lr, lr_final, beta1, beta2, adam_eps, seed)."""
+        tp = NbTrainParams()
+        self._lib.nb_default_train_params(C.byref(tp))
+        for k, v in overrides.items():
+            if not hasattr(tp, k):
+                raise TypeError(f"unknown training parameter {k}")
+            setattr(tp, k, v)
+        rep = NbTrainReport()
+        _check(self._lib, self._h, self._lib.nb_train(self._h, C.byref(tp), C.byref(rep)))
+        return dict(samples=rep.samples, loss_first=rep.loss_first, loss_last=rep.loss_last,
+                    gen_ms=rep.gen_ms, train_ms=rep.train_ms)
+
+    def set_model(self, blob) -> None:
+        blob = np.ascontiguousarray(blob, dtype=np.float32)
+        _check(self._lib, self._h, self._lib.nb_set_model(self._h, blob, len(blob)))
+
+    def get_model(self) -> np.ndarray:
+        out = np.zeros(_lib.NB_MODEL_FLOATS, np.float32)
+        _check(self._lib, self._h, self._lib.nb_get_model(self._h, out, len(out)))
+        return out
+
+    def coarse_features(self):
+        """(features [n, 24], coarse acceleration [n, 3], e2 [n, 3]) by body index."""
+        n = self.nbodies
+        f = np.zeros((n, _lib.NB_MLP_IN), np.float32)
+        a = np.zeros((n, 3), np.float32)
+        e2 = np.zeros((n, 3), np.float32)
+        _check(self._lib, self._h, self._lib.nb_coarse_features(self._h, ptr(f), ptr(a), ptr(e2)))
+        return f, a, e2
+
+    def mlp_forward(self, x) -> np.ndarray:
+        """Raw features [n, 24] -> residual [n, 3] through the installed network (BF16 matrix cores)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), 3), np.float32)
+        _check(self._lib, self._h, self._lib.nb_mlp_forward(self._h, x.reshape(-1), len(x), y.reshape(-1)))
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        """FP32 training kernels: (loss, gradient) for normalized x [n, 24], t [n, 3]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(_lib.NB_MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(self._lib, self._h, self._lib.nb_mlp_loss_grad(self._h, x.reshape(-1), t.reshape(-1), len(x),
+                                                              C.byref(loss), g))
+        return float(loss.value), g
+
+    def mlp_adam(self, x, t, iters: int, lr: float) -> np.ndarray:
+        """`iters` full-batch Adam steps from the installed parameters; returns the parameters."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        p = np.zeros(_lib.NB_MLP_PARAMS, np.float32)
+        _check(self._lib, self._h, self._lib.nb_mlp_adam(self._h, x.reshape(-1), t.reshape(-1), len(x),
+                                                         int(iters), float(lr), p))
+        return p

@@ -49,6 +49,7 @@ extern "C" {
 
 #define NB_MODE_PLAIN 0   /* plain Barnes-Hut at `theta` */
 #define NB_MODE_LEARNED 1 /* coarse traversal at `coarse_theta` + learned correction */
+#define NB_MODE_COARSE 2  /* the coarse traversal alone (no correction): the ablation */
 
 typedef struct nb_params {
     int32_t n_bodies;   /* NBODIES (-D NBODIES, java:212) */
@@ -176,6 +177,80 @@ int nb_direct_accel(nb_sim *sim, const int32_t *idx, int32_t ns, double *out3);
  * on nsamples distinct bodies drawn with `seed`: stats4 = (median, 90th
  * percentile, maximum, mean) of |a - a_direct| / |a_direct|. */
 int nb_force_error(nb_sim *sim, int32_t nsamples, uint64_t seed, double *stats4);
+
+/* ---- learned coarse-theta correction (no reference counterpart) ----------
+ *
+ * The coarse traversal opens cells at `coarse_theta` and adds, to the
+ * monopole of every accepted cell, its quadrupole under the softened kernel.
+ * It also emits 24 features per body in a frame built from the body's own
+ * coarse acceleration a (e1 = a/|a|, e2 from the quadrupole part, e3 = e1 x e2).
+ * A per-body MLP 24 -> 64 -> 64 -> 3 (tanh-GELU) maps the normalized features
+ * to the residual (a_fine - a) in that frame over |a|, where a_fine is the
+ * same traversal at `theta`. Inference runs with BF16 weights and activations
+ * on the matrix cores with FP32 accumulation; training runs in FP32 with
+ * Adam on pairs sampled from the simulation itself.
+ *
+ * Model blob (NB_MODEL_FLOATS floats): the NB_MLP_PARAMS network parameters
+ * W1[24][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3] (W[i][j] maps input
+ * i to output j), then the feature means mu_x[24] and standard deviations
+ * sd_x[24], then the target means mu_y[3] and standard deviations sd_y[3]:
+ * residual = (MLP((x - mu_x) / sd_x)) * sd_y + mu_y. */
+#define NB_MLP_IN 24
+#define NB_MLP_HIDDEN 64
+#define NB_MLP_OUT 3
+#define NB_MLP_PARAMS 5955
+#define NB_MODEL_FLOATS (NB_MLP_PARAMS + 2 * NB_MLP_IN + 2 * NB_MLP_OUT)
+
+typedef struct nb_train_params {
+    int32_t steps;          /* simulation steps that generate pairs (the bodies advance with plain theta) */
+    int32_t groups_per_step;/* vote groups of 64 bodies sampled per step */
+    int32_t iters;          /* Adam iterations */
+    int32_t batch;          /* minibatch size */
+    float lr;               /* Adam learning rate (cosine decay to lr_final) */
+    float lr_final;
+    float beta1, beta2, adam_eps;
+    uint64_t seed;
+    int32_t reserved[6];
+} nb_train_params;
+
+typedef struct nb_train_report {
+    int32_t samples;        /* training pairs collected */
+    float loss_first;       /* minibatch loss of the first and of the last iteration (normalized units) */
+    float loss_last;
+    double gen_ms;          /* pair generation, including the plain steps */
+    double train_ms;        /* the Adam iterations */
+    int64_t reserved[4];
+} nb_train_report;
+
+void nb_default_train_params(nb_train_params *tp);
+
+/* Generates pairs over tp->steps steps, trains the network from a seeded
+ * initialization and installs it; the simulation has advanced tp->steps
+ * steps with plain theta. rep may be NULL. */
+int nb_train(nb_sim *sim, const nb_train_params *tp, nb_train_report *rep);
+
+/* Installs / reads the model blob (NB_MODEL_FLOATS floats). */
+int nb_set_model(nb_sim *sim, const float *model, int32_t count);
+int nb_get_model(nb_sim *sim, float *model, int32_t count);
+
+/* Coarse-traversal features at the current positions (bounding box .. sort
+ * are run first): feat (24 per body), acc3 (the coarse acceleration, 3 per
+ * body) and e2 (3 per body), all by original body index; any may be NULL. */
+int nb_coarse_features(nb_sim *sim, float *feat, float *acc3, float *e2);
+
+/* The network alone, on host data. nb_mlp_forward: raw features x[n][24] ->
+ * residual y[n][3] through the installed model on the matrix-core path.
+ * nb_mlp_loss_grad: for the current FP32 parameters, normalized inputs
+ * x[n][24] and targets t[n][3], the mean squared error over n*3 outputs and
+ * its gradient (NB_MLP_PARAMS floats) from the FP32 training kernels.
+ * nb_mlp_adam: `iters` full-batch Adam steps on (x, t) with lr and the
+ * default betas, starting from the current parameters and zeroed moments;
+ * params receives the result. */
+int nb_mlp_forward(nb_sim *sim, const float *x, int32_t n, float *y);
+int nb_mlp_loss_grad(nb_sim *sim, const float *x, const float *t, int32_t n, float *loss,
+                     float *grad);
+int nb_mlp_adam(nb_sim *sim, const float *x, const float *t, int32_t n, int32_t iters, float lr,
+                float *params);
 
 /* Per-stage HIP-event timing of the following nb_step calls (0 = off). */
 int nb_set_timing(nb_sim *sim, int32_t enabled);

@@ -64,6 +64,7 @@ typedef struct orc_sim {
     float radius;
     float dq[ORC_MAXDEPTH];
     long long interactions;
+    double *cq;          /* 8*(nn+1): second moments of cells (learned mode), lazily */
 } orc_sim;
 
 /* GPUBarnesHutNBodySimulation.java:219-227, with maxComputeUnits = 16 (:126). */
@@ -131,6 +132,7 @@ orc_sim *orc_create(int n, float theta, float softening, float dt, int vote, int
 void orc_destroy(orc_sim *s)
 {
     if (!s) return;
+    free(s->cq);
     free(s->x); free(s->y); free(s->z); free(s->m);
     free(s->vx); free(s->vy); free(s->vz);
     free(s->ax); free(s->ay); free(s->az);
@@ -812,4 +814,238 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
             for (int j = 0; j < ORC_MLP_H; ++j) gW1[i * ORC_MLP_H + j] += (double)xs[i] * dz1[j];
     }
     return loss / ((double)n * ORC_MLP_OUT);
+}
+
+/* ---- coarse traversal of the learned correction (no reference counterpart)
+ *
+ * orc_moments: after summarize, every cell's second moments about its FP64
+ * centre of mass, bottom-up in ascending cell order (children first), over the
+ * compacted children in slot order: traceless Q = sum m (3 s s^T - |s|^2 I)
+ * (xx, yy, zz, xy, xz, yz) and trace T = sum m |s|^2, moved to the parent by
+ * the parallel-axis rule. Bodies are point masses at their FP32 positions.
+ *
+ * orc_coarse: the wave-uniform traversal of the force stage, restated per vote
+ * group with the opening table of `theta`: accepted cells contribute
+ *     m d / s^3 - Q d / s^5 + 5/2 (d.Q.d - T eps^2) d / s^7
+ * (d = centre of mass - body, s^2 = |d|^2 + eps^2, the second-order expansion
+ * of the softened kernel), bodies m d / s^3. Opening decisions use the FP32
+ * squared distance of the force stage; the terms are summed in FP64. For each
+ * body of the listed groups it writes the acceleration (acc, 3 per body) and,
+ * when feat is not NULL, the 24 features of the engine's coarse kernel and the
+ * frame vector e2 (3 per body). Output row of lane l of the i-th listed group:
+ * i * vote + l. */
+void orc_moments(orc_sim *s)
+{
+    const int n = s->n, nn = s->nn;
+    if (!s->cq) s->cq = (double *)calloc(8 * ((size_t)nn + 1), sizeof(double));
+    for (int node = s->bottom; node <= nn; ++node) {
+        const double *cc = s->com + 4 * (size_t)node;
+        double q[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 8; ++k) {
+            const int c = s->child[8 * node + k];
+            if (c < 0) continue;
+            double mx, px, py, pz;
+            if (c >= n) {
+                const double *c2 = s->com + 4 * (size_t)c;
+                mx = c2[3]; px = c2[0]; py = c2[1]; pz = c2[2];
+                for (int j = 0; j < 7; ++j) q[j] += s->cq[8 * (size_t)c + j];
+            } else {
+                mx = s->m[c]; px = s->x[c]; py = s->y[c]; pz = s->z[c];
+            }
+            const double dx = px - cc[0], dy = py - cc[1], dz = pz - cc[2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            q[0] += mx * (3.0 * dx * dx - r2);
+            q[1] += mx * (3.0 * dy * dy - r2);
+            q[2] += mx * (3.0 * dz * dz - r2);
+            q[3] += mx * 3.0 * dx * dy;
+            q[4] += mx * 3.0 * dx * dz;
+            q[5] += mx * 3.0 * dy * dz;
+            q[6] += mx * r2;
+        }
+        for (int j = 0; j < 7; ++j) s->cq[8 * (size_t)node + j] = q[j];
+        s->cq[8 * (size_t)node + 7] = 0.0;
+    }
+}
+
+/* second moments of a cell (for tests): out7 = Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, T */
+int orc_get_moments(const orc_sim *s, int cell, double *out7)
+{
+    if (!s->cq || cell < s->n || cell > s->nn) return -1;
+    for (int j = 0; j < 7; ++j) out7[j] = s->cq[8 * (size_t)cell + j];
+    return 0;
+}
+
+static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, float *feat, double *acc,
+                         double *e2out, long long *inter)
+{
+    const int n = s->n, nn = s->nn, W = s->vote;
+    const float *X = s->x, *Y = s->y, *Z = s->z, *M = s->m;
+    const int *child = s->child;
+    const float eps2 = s->eps2;
+    const int first = g * W;
+    const int cnt = (first + W <= n) ? W : n - first;
+    if (cnt <= 0) return;
+    float px[256], py[256], pz[256];
+    /* per lane: total, bodies, quadrupole part, u- and u^2-weighted cell monopoles */
+    double A[256][3], Bd[256][3], Qp[256][3], U1[256][3], U2[256][3], su[256], sv[256], smu[256];
+    double smc = 0.0, smb = 0.0;
+    long long nc = 0, nbod = 0;
+    int stackNode[ORC_MAXDEPTH + 1], stackPos[ORC_MAXDEPTH + 1];
+    for (int l = 0; l < cnt; ++l) {
+        const int b = s->sorted[first + l];
+        px[l] = X[b]; py[l] = Y[b]; pz[l] = Z[b];
+        for (int j = 0; j < 3; ++j) A[l][j] = Bd[l][j] = Qp[l][j] = U1[l][j] = U2[l][j] = 0.0;
+        su[l] = sv[l] = smu[l] = 0.0;
+    }
+    long long ni = 0;
+    int depth = 0;
+    stackNode[0] = nn;
+    stackPos[0] = 0;
+    while (depth >= 0) {
+        int top;
+        while ((top = stackPos[depth]) < 8) {
+            const int c = child[stackNode[depth] * 8 + top];
+            stackPos[depth] = top + 1;
+            if (c < 0) {
+                depth = depth - 1 > 0 ? depth - 1 : 0;
+                continue;
+            }
+            const float cxn = X[c], cyn = Y[c], czn = Z[c];
+            int accept = 1;
+            if (c >= n) {
+                for (int l = 0; l < cnt && accept; ++l) {
+                    const float dx = cxn - px[l], dy = cyn - py[l], dz = czn - pz[l];
+                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+                    if (!(d2 >= dq[depth])) accept = 0;
+                }
+            }
+            if (!accept) {
+                depth++;
+                stackNode[depth] = c;
+                stackPos[depth] = 0;
+                continue;
+            }
+            const double mc = (double)M[c];
+            const double *q = c >= n ? s->cq + 8 * (size_t)c : NULL;
+            if (c >= n) { smc += mc; ++nc; } else { smb += mc; ++nbod; }
+            for (int l = 0; l < cnt; ++l) {
+                const float fdx = cxn - px[l], fdy = cyn - py[l], fdz = czn - pz[l];
+                const float d2f = fmaf(fdx, fdx, fmaf(fdy, fdy, fmaf(fdz, fdz, eps2)));
+                const double dx = fdx, dy = fdy, dz = fdz, d2 = d2f;
+                const double r2i = 1.0 / d2, ri = sqrt(r2i), r3 = r2i * ri;
+                const double f = mc * r3;
+                A[l][0] += dx * f; A[l][1] += dy * f; A[l][2] += dz * f;
+                if (!q) {
+                    Bd[l][0] += dx * f; Bd[l][1] += dy * f; Bd[l][2] += dz * f;
+                    continue;
+                }
+                const double Qx = q[0] * dx + q[3] * dy + q[4] * dz;
+                const double Qy = q[3] * dx + q[1] * dy + q[5] * dz;
+                const double Qz = q[4] * dx + q[5] * dy + q[2] * dz;
+                const double dQd = Qx * dx + Qy * dy + Qz * dz;
+                const double r5 = r3 * r2i;
+                const double c2 = 2.5 * (dQd - q[6] * (double)eps2) * r5 * r2i;
+                const double gx = c2 * dx - r5 * Qx, gy = c2 * dy - r5 * Qy, gz = c2 * dz - r5 * Qz;
+                A[l][0] += gx; A[l][1] += gy; A[l][2] += gz;
+                Qp[l][0] += gx; Qp[l][1] += gy; Qp[l][2] += gz;
+                const double u = (double)dq[depth] * r2i;
+                U1[l][0] += u * f * dx; U1[l][1] += u * f * dy; U1[l][2] += u * f * dz;
+                U2[l][0] += u * u * f * dx; U2[l][1] += u * u * f * dy; U2[l][2] += u * u * f * dz;
+                su[l] += u;
+                sv[l] += u * u;
+                smu[l] += u * mc;
+            }
+            ni += cnt;
+        }
+        depth--;
+    }
+    *inter += ni;
+    /* group centroid and radius */
+    double gx = 0, gy = 0, gz = 0, rg = 0;
+    for (int l = 0; l < cnt; ++l) { gx += px[l]; gy += py[l]; gz += pz[l]; }
+    gx /= cnt; gy /= cnt; gz /= cnt;
+    for (int l = 0; l < cnt; ++l) {
+        const double ox = px[l] - gx, oy = py[l] - gy, oz = pz[l] - gz;
+        const double r = sqrt(ox * ox + oy * oy + oz * oz);
+        if (r > rg) rg = r;
+    }
+    for (int l = 0; l < cnt; ++l) {
+        const size_t o = (size_t)slot0 + l;
+        double *a = acc + 3 * o;
+        a[0] = A[l][0]; a[1] = A[l][1]; a[2] = A[l][2];
+        if (!feat) continue;
+        float *f = feat + ORC_MLP_IN * o;
+        for (int i = 0; i < ORC_MLP_IN; ++i) f[i] = 0.0f;
+        double e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0}, e3[3];
+        const double An = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        if (An > 0.0) {
+            const double inv = 1.0 / An;
+            for (int j = 0; j < 3; ++j) e1[j] = a[j] * inv;
+            const double *qv = Qp[l];
+            const double qe1 = qv[0] * e1[0] + qv[1] * e1[1] + qv[2] * e1[2];
+            double w[3] = {qv[0] - qe1 * e1[0], qv[1] - qe1 * e1[1], qv[2] - qe1 * e1[2]};
+            double Wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            if (!(Wn > 0.0)) {
+                /* the coordinate axis least aligned with e1 */
+                const double fx = fabs(e1[0]), fy = fabs(e1[1]), fz = fabs(e1[2]);
+                double t[3] = {0, 0, 0};
+                if (fx <= fy && fx <= fz) t[0] = 1; else if (fy <= fz) t[1] = 1; else t[2] = 1;
+                const double te1 = t[0] * e1[0] + t[1] * e1[1] + t[2] * e1[2];
+                for (int j = 0; j < 3; ++j) w[j] = t[j] - te1 * e1[j];
+                Wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+            }
+            for (int j = 0; j < 3; ++j) e2[j] = w[j] / Wn;
+            e3[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            e3[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            e3[2] = e1[0] * e2[1] - e1[1] * e2[0];
+#define ORC_LOC(V, I)                                                                              \
+    do {                                                                                           \
+        f[I] = (float)((V[0] * e1[0] + V[1] * e1[1] + V[2] * e1[2]) * inv);                        \
+        f[(I) + 1] = (float)((V[0] * e2[0] + V[1] * e2[1] + V[2] * e2[2]) * inv);                  \
+        f[(I) + 2] = (float)((V[0] * e3[0] + V[1] * e3[1] + V[2] * e3[2]) * inv);                  \
+    } while (0)
+            ORC_LOC(Bd[l], 0);
+            f[3] = (float)(qe1 * inv);
+            f[4] = (float)((qv[0] * e2[0] + qv[1] * e2[1] + qv[2] * e2[2]) * inv);
+            ORC_LOC(U1[l], 5);
+            ORC_LOC(U2[l], 8);
+            f[11] = (float)log1p(su[l]);
+            f[12] = (float)log1p(sv[l]);
+            f[13] = (float)log1p((double)nc);
+            f[14] = (float)log1p((double)nbod);
+            const double qn = sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]);
+            const double bn = sqrt(Bd[l][0] * Bd[l][0] + Bd[l][1] * Bd[l][1] + Bd[l][2] * Bd[l][2]);
+            f[15] = (float)log(fmax(qn * inv, 1e-30));
+            f[16] = (float)log(fmax(bn * inv, 1e-30));
+            f[17] = smc > 0.0 ? (float)(smu[l] / smc) : 0.0f;
+            f[18] = (smc + smb) > 0.0 ? (float)(smc / (smc + smb)) : 0.0f;
+            if (rg > 0.0) {
+                const double o3[3] = {px[l] - gx, py[l] - gy, pz[l] - gz};
+                f[19] = (float)((o3[0] * e1[0] + o3[1] * e1[1] + o3[2] * e1[2]) / rg);
+                f[20] = (float)((o3[0] * e2[0] + o3[1] * e2[1] + o3[2] * e2[2]) / rg);
+                f[21] = (float)((o3[0] * e3[0] + o3[1] * e3[1] + o3[2] * e3[2]) / rg);
+            }
+            f[22] = (float)log(fmax(rg / (double)s->radius, 1e-30));
+            const double mv[3] = {a[0] - Bd[l][0] - qv[0], a[1] - Bd[l][1] - qv[1], a[2] - Bd[l][2] - qv[2]};
+            const double mn = sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+            f[23] = (mn > 0.0 && bn > 0.0) ? (float)((mv[0] * Bd[l][0] + mv[1] * Bd[l][1] + mv[2] * Bd[l][2]) / (mn * bn)) : 0.0f;
+#undef ORC_LOC
+        }
+        if (e2out) {
+            e2out[3 * o] = e2[0]; e2out[3 * o + 1] = e2[1]; e2out[3 * o + 2] = e2[2];
+        }
+    }
+}
+
+long long orc_coarse(orc_sim *s, float theta, const int *groups, int ngroups, float *feat, double *acc,
+                     double *e2)
+{
+    float dq[ORC_MAXDEPTH];
+    if (!s->cq) orc_moments(s);
+    if (s->vote > 256 || s->max_depth > ORC_MAXDEPTH) return -1;
+    orc_dq_table(s->radius, theta * theta, s->eps2, s->max_depth, dq);
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int i = 0; i < ngroups; ++i) coarse_group(s, dq, groups[i], i * s->vote, feat, acc, e2, &total);
+    return total;
 }

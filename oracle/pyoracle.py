@@ -76,6 +76,15 @@ def lib():
         L.orc_direct_fp64.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, C.c_double, C.c_int,
                                       _i32p, _f64p, _f64p, _f64p]
         L.orc_num_threads.restype = C.c_int
+        L.orc_moments.argtypes = [vp]
+        L.orc_get_moments.argtypes = [vp, C.c_int, _f64p]
+        L.orc_get_moments.restype = C.c_int
+        L.orc_coarse.argtypes = [vp, C.c_float, _i32p, C.c_int, vp, _f64p, vp]
+        L.orc_coarse.restype = C.c_longlong
+        L.orc_mlp_params.restype = C.c_int
+        L.orc_mlp_forward.argtypes = [_f64p, _f32p, C.c_int, _f64p]
+        L.orc_mlp_loss_grad.argtypes = [_f64p, _f32p, _f32p, C.c_int, _f64p]
+        L.orc_mlp_loss_grad.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -116,11 +125,34 @@ def direct_fp64(x, y, z, m, softening, idx):
     return np.stack([ax, ay, az], axis=1)
 
 
+MLP_PARAMS = 5955
+
+
+def mlp_forward(params, x):
+    """FP64 MLP 24 -> 64 -> 64 -> 3 (tanh-GELU): y[n, 3] for x[n, 24]."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((len(x), 3), np.float64)
+    lib().orc_mlp_forward(P, x.reshape(-1), len(x), y.reshape(-1))
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    """(mean squared error over n*3 outputs, its gradient) in FP64."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = lib().orc_mlp_loss_grad(P, x.reshape(-1), t.reshape(-1), len(x), g)
+    return loss, g
+
+
 class Oracle:
     """One reference-semantics simulation on the CPU (mirrors the engine API)."""
 
     def __init__(self, n, theta=0.5, softening=0.05, dt=0.025, vote=16, flags=0):
         self.n = n
+        self.vote = vote
         self.h = lib().orc_create(n, theta, softening, dt, vote, flags)
         if not self.h:
             raise ValueError("orc_create rejected the parameters")
@@ -202,3 +234,32 @@ class Oracle:
         ch = np.zeros(8 * c, np.int32)
         lib().orc_get_cells(self.h, cnt, x, y, z, m, ch)
         return dict(count=cnt, x=x, y=y, z=z, m=m, child=ch.reshape(c, 8))
+
+    def moments(self):
+        """Compute every cell's second moments (after stage 3)."""
+        lib().orc_moments(self.h)
+
+    def cell_moments(self, cell):
+        out = np.zeros(7, np.float64)
+        if lib().orc_get_moments(self.h, int(cell), out) != 0:
+            raise ValueError("no moments for that node")
+        return out
+
+    def coarse(self, theta, groups=None, features=True):
+        """Coarse traversal (monopole + softened quadrupole) at `theta` over the
+        listed vote groups (all groups in order when None): returns (acc [rows, 3],
+        features [rows, 24] or None, e2 [rows, 3] or None, interactions); row
+        i * vote + l is lane l of the i-th group (sorted position when all)."""
+        if groups is None:
+            groups = np.arange((self.n + self.vote - 1) // self.vote, dtype=np.int32)
+        g = np.ascontiguousarray(groups, dtype=np.int32)
+        rows = len(g) * self.vote
+        acc = np.zeros((rows, 3), np.float64)
+        feat = np.zeros((rows, 24), np.float32) if features else None
+        e2 = np.zeros((rows, 3), np.float64) if features else None
+        ni = lib().orc_coarse(self.h, float(theta), g, len(g),
+                              None if feat is None else feat.ctypes.data_as(C.c_void_p), acc.reshape(-1),
+                              None if e2 is None else e2.ctypes.data_as(C.c_void_p))
+        if ni < 0:
+            raise RuntimeError("orc_coarse rejected the configuration")
+        return acc, feat, e2, int(ni)

@@ -68,7 +68,7 @@ def test_create_rejects_bad_params_without_gpu(nb, field, value):
 
 def test_learned_mode_needs_coarser_theta(nb):
     lib = nb.load()
-    p = nb.make_params(1000, theta=0.4, coarse_theta=0.3, mode=nb._lib.NB_MODE_LEARNED)
+    p = nb.make_params(1000, theta=0.4, coarse_theta=0.3, method=nb._lib.NB_MODE_LEARNED)
     h = C.c_void_p()
     assert lib.nb_create(C.byref(p), C.byref(h)) == nb._lib.NB_E_ARG
 

@@ -1,0 +1,255 @@
+"""GPU tests of the learned coarse-theta correction, through libnbody.so.
+
+Bars:
+  coarse traversal  accelerations within 1e-5 relative of the oracle's FP64
+                    restatement (same opening decisions, FP32 Kahan sums); the
+                    24 features within 1e-3 absolute + 1e-3 relative on all but
+                    a handful of bodies (FP32 sums against FP64)
+  BF16 inference    the matrix-core network against the oracle's FP64 forward
+                    pass on the same normalized inputs: rms difference <= 1% of
+                    the rms output, max <= 5% of the max output (BF16 weights and
+                    activations, FP32 accumulation)
+  training kernels  FP32 gradient within 1e-4 of the largest FP64 component;
+                    first Adam step = -lr sign(g) (closed form); four steps
+                    against an FP64 Adam restatement driven by oracle gradients
+  ablation          after training on the simulation's own pairs, the learned
+                    method's median error against the FP64 direct sum is below
+                    the coarse traversal's alone
+  fallback          a check tolerance nobody meets makes the step plain theta
+"""
+import numpy as np
+import pytest
+
+from helpers import make_oracle, make_sim, rel_err
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.01
+THETA_C = 0.9
+
+
+def _coarse_sim(st, method=None, **kw):
+    import nbody_amd._lib as L
+
+    return make_sim(st, theta=0.4, softening=EPS, dt=1e-3, vote_width=64, coarse_theta=THETA_C,
+                    method=L.NB_MODE_COARSE if method is None else method, **kw)
+
+
+def _random_model(seed, scale_out=1e-3):
+    rng = np.random.default_rng(seed)
+    P = np.concatenate([rng.uniform(-1, 1, 24 * 64) / np.sqrt(24), rng.uniform(-1, 1, 64) / np.sqrt(24),
+                        rng.uniform(-1, 1, 64 * 64) / 8, rng.uniform(-1, 1, 64) / 8,
+                        rng.uniform(-1, 1, 64 * 3) / 8, rng.uniform(-1, 1, 3) / 8])
+    xmu = rng.normal(size=24)
+    xsd = rng.uniform(0.5, 2.0, 24)
+    ymu = rng.normal(size=3) * scale_out
+    ysd = rng.uniform(0.5, 2.0, 3) * scale_out
+    return np.concatenate([P, xmu, xsd, ymu, ysd]).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["plummer_16k", "clusters_32k", "ragged_5000"])
+def test_coarse_features_match_oracle(nb, oracle, case):
+    from nbody_amd import universe as U
+
+    st = {"plummer_16k": lambda: U.plummer(16384, seed=2),
+          "clusters_32k": lambda: U.gaussian_clusters(32768, seed=4),
+          "ragged_5000": lambda: U.uniform_ball(5000, seed=9)}[case]()
+    sim = _coarse_sim(st)
+    f, a, e2 = sim.coarse_features()
+    orc = make_oracle(st, 0.4, EPS, 1e-3, 64, flags=1)
+    assert orc.run_stages(1, 4) == 0
+    orc.moments()
+    srt = orc.sorted()
+    assert np.array_equal(srt, sim.sorted())
+    oa, of, oe2, _ = orc.coarse(THETA_C)
+    n = len(st[0])
+    oa, of, oe2 = oa[:n], of[:n], oe2[:n]
+    # engine outputs are by body index, oracle rows by sorted position
+    ga, gf, ge2 = a[srt], f[srt], e2[srt]
+    err = rel_err(ga, oa)
+    assert err.max() <= 1e-5, (err.max(), int(err.argmax()))
+    bad = np.abs(gf - of) > 1e-3 + 1e-3 * np.abs(of)
+    rows_bad = bad.any(axis=1)
+    assert rows_bad.sum() <= max(2, n // 2000), (rows_bad.sum(), np.argwhere(bad)[:10])
+    sim.close()
+
+
+def test_coarse_mode_force_stage_matches_oracle(nb, oracle):
+    from nbody_amd import universe as U
+
+    st = U.plummer(20000, seed=12)
+    sim = _coarse_sim(st)
+    for s in range(1, 6):
+        sim.run_stage(s)
+    g = sim.accel()
+    orc = make_oracle(st, 0.4, EPS, 1e-3, 64, flags=1)
+    assert orc.run_stages(1, 4) == 0
+    orc.moments()
+    srt = orc.sorted()
+    oa = orc.coarse(THETA_C, features=False)[0][: len(srt)]
+    err = rel_err(g[srt], oa)
+    assert err.max() <= 1e-5, err.max()
+    sim.close()
+
+
+def test_mlp_bf16_inference_vs_oracle(nb, oracle):
+    """Also pins the matrix-instruction lane layout: a wrong A/B/D mapping
+    moves outputs by O(1), far outside the BF16 bar."""
+    from nbody_amd import universe as U
+
+    st = U.uniform_ball(4096, seed=3)
+    sim = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED)
+    blob = _random_model(5)
+    sim.set_model(blob)
+    assert np.array_equal(sim.get_model(), blob)
+    rng = np.random.default_rng(6)
+    xmu, xsd = blob[5955:5979], blob[5979:6003]
+    ymu, ysd = blob[6003:6006].astype(np.float64), blob[6006:6009].astype(np.float64)
+    for rows in (1000, 4096, 77):
+        x = (xmu + xsd * rng.normal(size=(rows, 24))).astype(np.float32)
+        y = sim.mlp_forward(x)
+        xn = ((x - xmu) * (np.float32(1) / xsd)).astype(np.float32)
+        ref = oracle.mlp_forward(blob[:5955].astype(np.float64), xn) * ysd + ymu
+        d = y - ref
+        assert np.sqrt((d ** 2).mean()) <= 0.01 * np.sqrt((ref ** 2).mean()), (rows, np.sqrt((d ** 2).mean()))
+        assert np.abs(d).max() <= 0.05 * np.abs(ref).max()
+    sim.close()
+
+
+def test_training_gradient_vs_oracle(nb, oracle):
+    from nbody_amd import universe as U
+
+    sim = _coarse_sim(U.uniform_ball(2048, seed=3), method=nb._lib.NB_MODE_LEARNED)
+    blob = _random_model(7)
+    sim.set_model(blob)
+    rng = np.random.default_rng(8)
+    for n in (512, 1000):
+        x = rng.normal(size=(n, 24)).astype(np.float32)
+        t = rng.normal(size=(n, 3)).astype(np.float32)
+        loss, g = sim.mlp_loss_grad(x, t)
+        oloss, og = oracle.mlp_loss_grad(blob[:5955].astype(np.float64), x, t)
+        assert abs(loss - oloss) <= 1e-5 * abs(oloss)
+        assert np.abs(g - og).max() <= 1e-4 * np.abs(og).max(), np.abs(g - og).max()
+    sim.close()
+
+
+def test_adam_steps(nb, oracle):
+    from nbody_amd import universe as U
+
+    sim = _coarse_sim(U.uniform_ball(2048, seed=3), method=nb._lib.NB_MODE_LEARNED)
+    blob = _random_model(9)
+    sim.set_model(blob)
+    rng = np.random.default_rng(10)
+    x = rng.normal(size=(256, 24)).astype(np.float32)
+    t = rng.normal(size=(256, 3)).astype(np.float32)
+    p0 = blob[:5955].astype(np.float64)
+    lr = 1e-3
+    _, g0 = oracle.mlp_loss_grad(p0, x, t)
+    p1 = sim.mlp_adam(x, t, 1, lr)
+    big = np.abs(g0) > 1e-5
+    # first step from zero moments: -lr * g / (|g| + eps) = -lr sign(g)
+    assert np.allclose((p1 - blob[:5955])[big], -lr * np.sign(g0[big]), rtol=1e-3, atol=1e-7)
+    # four steps against an FP64 Adam restatement on oracle gradients
+    p, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    for k in range(1, 5):
+        _, g = oracle.mlp_loss_grad(p, x, t)
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        p = p - lr * (m / (1 - 0.9 ** k)) / (np.sqrt(v / (1 - 0.999 ** k)) + 1e-8)
+    p4 = sim.mlp_adam(x, t, 4, lr)
+    assert np.abs(p4 - p).max() <= 0.05 * 4 * lr, np.abs(p4 - p).max()
+    sim.close()
+
+
+def test_model_validation(nb):
+    from nbody_amd import universe as U
+
+    sim = _coarse_sim(U.uniform_ball(2048, seed=3), method=nb._lib.NB_MODE_LEARNED)
+    with pytest.raises(nb.NbError) as e:
+        sim.step()                      # learned mode without a model
+    assert e.value.code == nb._lib.NB_E_MODEL
+    blob = _random_model(1)
+    bad = blob.copy()
+    bad[6000] = -1.0                    # a negative feature scale
+    with pytest.raises(nb.NbError):
+        sim.set_model(bad)
+    bad = blob.copy()
+    bad[10] = np.nan
+    with pytest.raises(nb.NbError):
+        sim.set_model(bad)
+    with pytest.raises(nb.NbError):
+        sim.set_model(blob[:-1])
+    sim.close()
+
+
+def test_learned_probe_equals_step_and_fused_equals_unfused(nb):
+    from nbody_amd import universe as U
+
+    st = U.plummer(30000, seed=13)
+    blob = _random_model(2)
+    sims = []
+    for flags in (0, nb._lib.NB_FUSED_UPDATE):
+        s = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED, flags=flags)
+        s.set_model(blob)
+        sims.append(s)
+    probe = sims[0].probe_accel()
+    for s in sims:
+        s.steps(2)
+    a0, a1 = sims[0].accel(), sims[1].accel()
+    assert np.array_equal(a0, a1)
+    x0, x1 = sims[0].state(), sims[1].state()
+    for u, w in zip(x0, x1):
+        assert np.allclose(u, w, rtol=1e-6, atol=1e-7)
+    # the probe ran on the initial positions, as the first step's force did
+    sim = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED)
+    sim.set_model(blob)
+    sim.steps(1)
+    assert np.array_equal(sim.accel(), probe)
+    for s in sims + [sim]:
+        s.close()
+
+
+def test_check_fallback_to_plain(nb):
+    from nbody_amd import universe as U
+
+    st = U.plummer(20000, seed=14)
+    blob = _random_model(3)
+    strict = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED, check_tol=1e-12, check_every=1, check_samples=256)
+    strict.set_model(blob)
+    strict.steps(2)
+    c = strict.counters()
+    assert c.fallbacks == 1 and c.last_check_median > 1e-12
+    plain = make_sim(st, theta=0.4, softening=EPS, dt=1e-3, vote_width=64)
+    plain.steps(2)
+    for u, w in zip(strict.state(), plain.state()):
+        assert np.allclose(u, w, rtol=1e-5, atol=1e-6)
+    assert np.allclose(strict.accel(), plain.accel(), rtol=1e-4, atol=1e-6)
+    loose = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED, check_tol=10.0, check_every=1, check_samples=256)
+    loose.set_model(blob)
+    loose.steps(2)
+    c = loose.counters()
+    assert c.fallbacks == 0 and 0.0 < c.last_check_median < 10.0
+    for s in (strict, plain, loose):
+        s.close()
+
+
+def test_training_ablation(nb):
+    """What the trained network gains over the coarse traversal alone, on the
+    simulation that produced its training pairs."""
+    from nbody_amd import universe as U
+
+    st = U.gaussian_clusters(131072, seed=4)
+    sim = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED)
+    rep = sim.train(steps=6, groups_per_step=256, iters=1500, batch=4096, seed=3)
+    assert rep["samples"] == 6 * 256 * 64
+    assert rep["loss_last"] < 0.9 * rep["loss_first"], rep
+    learned = sim.force_error(4096, seed=77)
+    x, y, z, vx, vy, vz = sim.state()
+    m = st[6]
+    coarse = _coarse_sim((x, y, z, vx, vy, vz, m))
+    alone = coarse.force_error(4096, seed=77)
+    print(f"ablation: coarse alone median {alone['median']:.3e}, learned {learned['median']:.3e}, "
+          f"train {rep}")
+    assert learned["median"] < 0.95 * alone["median"], (learned, alone)
+    sim.close()
+    coarse.close()
