@@ -233,23 +233,27 @@ def test_check_fallback_to_plain(nb):
         s.close()
 
 
+@pytest.mark.slow
 def test_training_ablation(nb):
-    """What the trained network gains over the coarse traversal alone, on the
-    simulation that produced its training pairs."""
+    """What the trained network gains over the coarse traversal alone, at the
+    benchmark's size (4,194,304 clustered bodies, coarse theta 0.9), on the
+    simulation that produced its training pairs. (At 131,072 bodies the gain
+    is about 2 %: the residual left by the quadrupole is then mostly the
+    Poisson noise of sparsely filled cells, which no per-body feature sees.)"""
     from nbody_amd import universe as U
 
-    st = U.gaussian_clusters(131072, seed=4)
+    st = U.gaussian_clusters(4_194_304, seed=4)
     sim = _coarse_sim(st, method=nb._lib.NB_MODE_LEARNED)
-    rep = sim.train(steps=6, groups_per_step=256, iters=1500, batch=4096, seed=3)
-    assert rep["samples"] == 6 * 256 * 64
-    assert rep["loss_last"] < 0.9 * rep["loss_first"], rep
+    rep = sim.train(steps=6, groups_per_step=512, iters=2000, batch=4096, seed=3)
+    assert rep["samples"] == 6 * 512 * 64
+    assert rep["loss_last"] < 0.8 * rep["loss_first"], rep
     learned = sim.force_error(4096, seed=77)
     x, y, z, vx, vy, vz = sim.state()
-    m = st[6]
-    coarse = _coarse_sim((x, y, z, vx, vy, vz, m))
+    coarse = _coarse_sim((x, y, z, vx, vy, vz, st[6]))
     alone = coarse.force_error(4096, seed=77)
-    print(f"ablation: coarse alone median {alone['median']:.3e}, learned {learned['median']:.3e}, "
-          f"train {rep}")
-    assert learned["median"] < 0.95 * alone["median"], (learned, alone)
+    print(f"ablation: coarse alone median {alone['median']:.3e} p90 {alone['p90']:.3e}, "
+          f"learned median {learned['median']:.3e} p90 {learned['p90']:.3e}, train {rep}")
+    assert learned["median"] < 0.93 * alone["median"], (learned, alone)
+    assert learned["p90"] < alone["p90"], (learned, alone)
     sim.close()
     coarse.close()
