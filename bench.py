@@ -68,15 +68,15 @@ def make_sim(state, mode, theta_coarse=0.9):
     return sim
 
 
-def train_learned(state):
+def train_learned(state, candidates, train):
     """Train one network per candidate coarse angle (on the GPU, on the
     simulation's own pairs) and keep the coarsest whose error on a calibration
     sample meets the target."""
     chosen = None
-    for th in THETA_COARSE_CANDIDATES:
+    for th in candidates:
         sim = make_sim(state, "learned", th)
         t0 = time.perf_counter()
-        rep = sim.train(**TRAIN)
+        rep = sim.train(**train)
         rep["wall_s"] = time.perf_counter() - t0
         cal = sim.force_error(4096, seed=4242)
         log(f"[learned] coarse theta {th}: calibration median {cal['median']:.3e}, train {rep}")
@@ -151,6 +151,11 @@ def main():
     ap.add_argument("--mode", choices=["learned", "plain"], default=None)
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--theta-coarse", type=float, default=None,
+                    help="use this coarse angle instead of calibrating one")
+    ap.add_argument("--train-iters", type=int, default=None, help="Adam iterations (default 4000)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run the vertex copy kernel once per mode (a copy of known size for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -171,7 +176,11 @@ def main():
     learned_info = None
     for m in (["plain", "learned"] if mode == "learned" else ["plain"]):
         if m == "learned":
-            sim, th, rep, cal = train_learned(state)
+            train = dict(TRAIN)
+            if args.train_iters is not None:
+                train["iters"] = args.train_iters
+            cands = (args.theta_coarse,) if args.theta_coarse else THETA_COARSE_CANDIDATES
+            sim, th, rep, cal = train_learned(state, cands, train)
             learned_info = dict(theta_coarse=th, train=rep, calibration_median=cal["median"])
         else:
             sim = make_sim(state, "plain")
@@ -183,6 +192,11 @@ def main():
         err = sim.force_error(args.samples, seed=1000)
         results[m] = dict(steps_per_s=sps, ms_per_step=ms, stages=stages, interactions=inter, err=err,
                           fallbacks=fallbacks)
+        if args.copy_calibration:
+            # k_copy_vertices reads 2 x 16 B and writes 2 x 16 B per body
+            pos4 = np.zeros((args.n, 4), np.float32)
+            vel4 = np.zeros((args.n, 4), np.float32)
+            sim.copy_vertices(pos4, vel4)
         log(f"[{m}] {sps:.3f} steps/s, {ms:.2f} ms/step, median err {err['median']:.2e}, stages {stages}")
         sim.close()
 

@@ -7,7 +7,15 @@ namespace nb {
 
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 
+constexpr float kFar = __builtin_huge_valf();
 constexpr int kStack = 512; // >= 7 * 64 + 8 entries: the deepest possible push pattern
+
+// The opening vote of a wave: true when every active lane's predicate holds.
+// The compare result goes straight into the lane mask (no bool round trip).
+__device__ __forceinline__ bool wave_all(bool pred)
+{
+    return __builtin_amdgcn_ballot_w64(pred) == __builtin_amdgcn_read_exec();
+}
 
 __device__ __forceinline__ void kahan_add(float &sum, float &comp, float x)
 {

@@ -83,11 +83,12 @@ int fail(nb_sim *s, int code, const char *fmt, ...)
 int build_threads(int n) { return std::min(131072, std::max(256, (n / 16 + 255) / 256 * 256)); }
 
 // Cell pool: the reference's NUMBER_OF_NODES = 2N (java:219-227) scaled by
-// pool_factor / 2, plus 8 cells per build thread for unpublished spares.
+// pool_factor / 2, plus 20 cells per build thread for the unused rest of its
+// allocation chunk (at most 15 of 16) and an unpublished spare.
 long long pool_cells(const nb_params *p)
 {
     const double base = std::max(16384.0, std::ceil((double)p->pool_factor * p->n_bodies));
-    return (long long)base + 8LL * build_threads(p->n_bodies);
+    return (long long)base + 20LL * build_threads(p->n_bodies);
 }
 
 int validate(const nb_params *p)
@@ -392,6 +393,7 @@ int train_buffers(Scratch &m, nb::Train &t, int S, int B, int iters)
     for (float **f : {&t.Z1, &t.H1, &t.Z2, &t.H2, &t.dZ1, &t.dZ2})
         if (!r) r = m.get(f, (size_t)B * nb::kHid);
     if (!r) r = m.get(&t.dY, (size_t)B * nb::kOut);
+    if (!r) r = m.get(&t.gpart, (size_t)nb::kGradChunks * (nb::kHid + 1) * nb::kHid);
     if (!r) r = m.get(&t.loss_part, (size_t)(B + 255) / 256);
     if (!r) r = m.get(&t.loss_hist, (size_t)std::max(iters, 1));
     t.S = S;

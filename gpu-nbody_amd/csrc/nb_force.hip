@@ -48,7 +48,9 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
 
     const int pos = blockIdx.x * 256 + tid;
     const bool valid = pos < n;
-    const float4 me = valid ? bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // a lane past the last body sits at infinity: its d^2 is infinite, so it
+    // accepts every cell and never holds up the vote (its sums are discarded)
+    const float4 me = valid ? bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
 
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
             const float dx = q[k].x - me.x, dy = q[k].y - me.y, dz = q[k].z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             bool acc = v < n;
-            if (!acc) acc = __ballot(!valid || d2 >= thr) == __ballot(1);
+            if (!acc) acc = wave_all(d2 >= thr);
             if (acc) {
                 const float ri = __builtin_amdgcn_rsqf(d2);
                 const float f = q[k].w * (ri * ri * ri);
@@ -139,7 +141,9 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
 
     const int pos = blockIdx.x * 256 + tid;
     const bool valid = pos < n;
-    const float4 me = valid ? d.bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // a lane past the last body sits at infinity: its d^2 is infinite, so it
+    // accepts every cell and never holds up the vote (its sums are discarded)
+    const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const unsigned long long gmask = ((1ull << G) - 1ull) << ((lane / G) * G);
     const bool leader = (lane % G) == 0;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
             const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             bool acc = v < n;
-            if (!acc) acc = (__ballot(!valid || d2 >= thr) & gmask) == gmask;
+            if (!acc) acc = (__builtin_amdgcn_ballot_w64(d2 >= thr) & gmask) == gmask;
             if (acc) {
                 const float ri = __builtin_amdgcn_rsqf(d2);
                 const float f = q.w * (ri * ri * ri);

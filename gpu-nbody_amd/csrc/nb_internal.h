@@ -20,8 +20,8 @@
 //            cq64  : double[8*cap]  second moments about the centre of mass, learned
 //                                   mode only: traceless Q (xx, yy, zz, xy, xz, yz),
 //                                   trace T, 0
-//            cqa/cqb: float4/float2[8*cap] each child's (Qxx, Qyy, Qxy, Qxz) and
-//                                   (Qyz, T eps^2), next to cdata
+//            cqa/cqb: float4[8*cap] each child's (Qxx, Qyy, Qzz, T eps^2) and
+//                                   (Qxy, Qxz, Qyz, 0), next to cdata
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
 //            sorted : int32[N]      body index at each sorted position
@@ -93,7 +93,7 @@ struct Dev {
     float4 *cdata;
     double *cq64;
     float4 *cqa;
-    float2 *cqb;
+    float4 *cqb;
     int *lvl_list;
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
     float4 *bnode;
@@ -133,6 +133,11 @@ struct Mlp {
     const float *b1, *b2, *b3;
     const float *xmu, *xrs;
 };
+// normalized features and training targets are clipped to [-kClipZ, kClipZ]:
+// both have rare tails of tens of standard deviations (bodies whose coarse
+// acceleration nearly cancels), which would otherwise dominate the squared loss
+constexpr float kClipZ = 8.0f;
+constexpr int kGradChunks = 16; // row chunks of the weight-gradient reduction
 constexpr size_t kMlpBufBytes = (64 * 32 + 64 * 64 + 16 * 64) * 2 + (64 + 64 + 16 + 24 + 24) * 4;
 
 struct Learn {
@@ -168,6 +173,7 @@ struct Train {
     const float *X, *T;    // dataset: normalized features [S][24], targets [S][3]
     int S;
     float *Z1, *H1, *Z2, *H2, *dZ1, *dZ2, *dY; // [B][64] (dY [B][3])
+    float *gpart;          // weight-gradient partial sums: kGradChunks x (kHid + 1) x kHid
     float *loss_part;      // one per 256 rows of the batch
     float *loss_hist;      // one per iteration
 };

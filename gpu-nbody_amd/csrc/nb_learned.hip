@@ -76,6 +76,8 @@ __device__ __forceinline__ float gelu_grad(float x)
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x2);
 }
 
+__device__ __forceinline__ float clipz(float z) { return fminf(fmaxf(z, -kClipZ), kClipZ); }
+
 __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t *p)
 {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(p));
@@ -88,7 +90,7 @@ template <int OUT>
 __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
                                                   const float4 *__restrict__ cdata,
                                                   const float4 *__restrict__ cqa,
-                                                  const float2 *__restrict__ cqb,
+                                                  const float4 *__restrict__ cqb,
                                                   const float *__restrict__ dqtab,
                                                   const int *__restrict__ groups, int ngroups,
                                                   float *__restrict__ xraw, float4 *__restrict__ frame_out,
@@ -121,7 +123,9 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
     const int pos = g * 64 + lane;
     const bool valid = pos >= 0 && pos < n;
     if (__ballot(valid) == 0ull) return; // wave-uniform
-    const float4 me = valid ? d.bnode[pos] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // a lane past the last body sits at infinity: its d^2 is infinite, so it
+    // accepts every cell and never holds up the vote (its sums are discarded)
+    const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
 
@@ -151,7 +155,10 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
             ch[k] = __builtin_amdgcn_readfirstlane(chp[k]);
             q[k] = cdp[k];
         }
-        float pbx = 0.f, pby = 0.f, pbz = 0.f, pcx = 0.f, pcy = 0.f, pcz = 0.f;
+        // per popped cell: bodies, cell monopoles and cell quadrupoles summed
+        // separately (the features need the parts), then one compensated add
+        float pbx = 0.f, pby = 0.f, pbz = 0.f, pmx = 0.f, pmy = 0.f, pmz = 0.f;
+        float pqx = 0.f, pqy = 0.f, pqz = 0.f;
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -170,31 +177,27 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                     ++nbod;
                     smb += q[k].w;
                 }
-            } else if (__ballot(!valid || d2 >= thr) == __ballot(1)) {
-                const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qxy, Qxz)
-                const float2 qb = cqb[(size_t)c * 8 + k]; // (Qyz, T eps^2)
+            } else if (wave_all(d2 >= thr)) {
+                const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qzz, T eps^2)
+                const float4 qb = cqb[(size_t)c * 8 + k]; // (Qxy, Qxz, Qyz, 0)
                 const float ri = __builtin_amdgcn_rsqf(d2);
                 const float r2i = ri * ri;
                 const float r3 = r2i * ri;
                 const float f = q[k].w * r3;
-                const float qzz = -(qa.x + qa.y);
-                const float Qx = fmaf(qa.x, dx, fmaf(qa.z, dy, qa.w * dz));
-                const float Qy = fmaf(qa.z, dx, fmaf(qa.y, dy, qb.x * dz));
-                const float Qz = fmaf(qa.w, dx, fmaf(qb.x, dy, qzz * dz));
+                const float Qx = fmaf(qa.x, dx, fmaf(qb.x, dy, qb.y * dz));
+                const float Qy = fmaf(qb.x, dx, fmaf(qa.y, dy, qb.z * dz));
+                const float Qz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.z * dz));
                 const float dQd = fmaf(Qx, dx, fmaf(Qy, dy, Qz * dz));
                 const float r5 = r3 * r2i;
-                const float c2 = 2.5f * (dQd - qb.y) * (r5 * r2i);
-                const float gx = fmaf(c2, dx, -r5 * Qx);
-                const float gy = fmaf(c2, dy, -r5 * Qy);
-                const float gz = fmaf(c2, dz, -r5 * Qz);
-                pcx += fmaf(dx, f, gx);
-                pcy += fmaf(dy, f, gy);
-                pcz += fmaf(dz, f, gz);
+                const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
                 ++cnt;
                 if (kFeatOn) {
-                    qx += gx;
-                    qy += gy;
-                    qz += gz;
+                    pmx = fmaf(dx, f, pmx);
+                    pmy = fmaf(dy, f, pmy);
+                    pmz = fmaf(dz, f, pmz);
+                    pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
+                    pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
+                    pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
                     const float u = thr * r2i;
                     const float fu = f * u;
                     ux = fmaf(fu, dx, ux);
@@ -209,6 +212,12 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                     smu = fmaf(u, q[k].w, smu);
                     smc += q[k].w;
                     ++nc;
+                } else {
+                    // monopole and quadrupole radial terms share one factor
+                    const float h = f + c2;
+                    pmx = fmaf(dx, h, fmaf(-r5, Qx, pmx));
+                    pmy = fmaf(dy, h, fmaf(-r5, Qy, pmy));
+                    pmz = fmaf(dz, h, fmaf(-r5, Qz, pmz));
                 }
             } else {
                 // the tree is at most kMaxDepth deep: anything else is corrupt
@@ -220,13 +229,16 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                 ++sp;
             }
         }
-        kahan_add(sx, kx, pbx + pcx);
-        kahan_add(sy, ky, pby + pcy);
-        kahan_add(sz, kz, pbz + pcz);
+        kahan_add(sx, kx, (pbx + pmx) + pqx);
+        kahan_add(sy, ky, (pby + pmy) + pqy);
+        kahan_add(sz, kz, (pbz + pmz) + pqz);
         if (kFeatOn) {
             bx += pbx;
             by += pby;
             bz += pbz;
+            qx += pqx;
+            qy += pqy;
+            qz += pqz;
         }
         ninter += cnt;
         if (bad) break;
@@ -330,8 +342,8 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                 uint32_t w[kFeat / 2];
 #pragma unroll
                 for (int i = 0; i < kFeat; i += 2) {
-                    const uint32_t lo = to_bf16((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]);
-                    const uint32_t hi = to_bf16((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]);
+                    const uint32_t lo = to_bf16(clipz((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]));
+                    const uint32_t hi = to_bf16(clipz((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]));
                     w[i / 2] = lo | (hi << 16);
                 }
                 uint4 *o = reinterpret_cast<uint4 *>(L.feat + (size_t)pos * kFeat);
@@ -358,7 +370,7 @@ hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqt
     if (waves <= 0) return hipSuccess;
     const dim3 grid((waves + 3) / 4), block(256);
 #define NB_COARSE_ARGS                                                                             \
-    d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float2 *)d.cqb, \
+    d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float4 *)d.cqb, \
         dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
     switch (out) {
     case kOutInfer: hipLaunchKernelGGL(k_coarse64<kOutInfer>, grid, block, 0, st, NB_COARSE_ARGS); break;
@@ -504,7 +516,7 @@ __global__ void k_norm_bf16(Mlp m, const float *__restrict__ x, int rows, uint16
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows * kFeat) return;
     const int c = i % kFeat;
-    out[i] = to_bf16((x[i] - m.xmu[c]) * m.xrs[c]);
+    out[i] = to_bf16(clipz((x[i] - m.xmu[c]) * m.xrs[c]));
 }
 
 hipError_t launch_norm_bf16(const Learn &L, const float *xraw, int rows, uint16_t *out, hipStream_t st)
@@ -585,7 +597,7 @@ __global__ void k_apply_standardize(float *__restrict__ x, int rows, int cols, c
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)rows * cols) return;
     const int c = (int)(i % cols);
-    x[i] = (x[i] - mu[c]) / sd[c];
+    x[i] = clipz((x[i] - mu[c]) / sd[c]);
 }
 
 hipError_t launch_standardize(float *x, int rows, int cols, float *mu, float *sd, hipStream_t st)
@@ -730,20 +742,23 @@ __global__ __launch_bounds__(256) void k_back_hidden(const float *__restrict__ d
     }
 }
 
-// gW[k][j] = sum_r A[r][k] dZ[r][j] (block k < K), gb[j] = sum_r dZ[r][j] (block K).
-// Each thread sums every 4th row; the 4 partial sums are added in a fixed order.
+// Partial weight gradients: block (k, ch) sums rows [ch B / C, (ch + 1) B / C)
+// of A[r][k] dZ[r][j] (k < K), or of dZ[r][j] for the bias (k = K); each
+// thread takes every 4th row and the 4 partial sums are added in a fixed
+// order. k_wgrad_reduce then adds the C chunks in order: deterministic.
 template <bool GATHER>
 __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A, int lda, uint64_t seed, int it, int S,
                                                const float *__restrict__ dZ, int N, int B, int K,
-                                               float *__restrict__ gW, float *__restrict__ gb)
+                                               float *__restrict__ part)
 {
     __shared__ float red[4][64];
     const int t = threadIdx.x;
     const int j = t & 63, rq = t >> 6;
-    const int k = blockIdx.x;
+    const int k = blockIdx.x, ch = blockIdx.y;
+    const int r0 = (int)((long long)B * ch / kGradChunks), r1 = (int)((long long)B * (ch + 1) / kGradChunks);
     float acc = 0.f;
     if (j < N) {
-        for (int r = rq; r < B; r += 4) {
+        for (int r = r0 + rq; r < r1; r += 4) {
             float a = 1.0f;
             if (k < K) a = GATHER ? A[(size_t)batch_row(seed, it, r, S) * lda + k] : A[(size_t)r * lda + k];
             acc = fmaf(a, dZ[(size_t)r * N + j], acc);
@@ -751,13 +766,22 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ A, int 
     }
     red[rq][j] = acc;
     __syncthreads();
-    if (rq == 0 && j < N) {
-        const float v = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
-        if (k < K)
-            gW[k * N + j] = v;
-        else
-            gb[j] = v;
-    }
+    if (rq == 0 && j < N)
+        part[((size_t)ch * (K + 1) + k) * N + j] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+}
+
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int K, int N, float *__restrict__ gW,
+                               float *__restrict__ gb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (K + 1) * N) return;
+    float v = 0.f;
+    for (int ch = 0; ch < kGradChunks; ++ch) v += part[(size_t)ch * (K + 1) * N + i];
+    const int k = i / N, j = i % N;
+    if (k < K)
+        gW[k * N + j] = v;
+    else
+        gb[j] = v;
 }
 
 __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
@@ -796,14 +820,20 @@ hipError_t launch_train_iter(const Train &t, int B, int it, uint64_t seed, bool 
                        it, t.S, B, t.dY, t.loss_part);
     hipLaunchKernelGGL(k_back_hidden<kOut>, gh, blk, 0, st, (const float *)t.dY, P + kOffW3,
                        (const float *)t.Z2, B, t.dZ2);
-    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1), blk, 0, st, (const float *)t.H2, kHid, seed, it, t.S,
-                       (const float *)t.dY, kOut, B, kHid, G + kOffW3, G + kOffB3);
+    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1, kGradChunks), blk, 0, st, (const float *)t.H2, kHid, seed, it,
+                       t.S, (const float *)t.dY, kOut, B, kHid, t.gpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(((kHid + 1) * kOut + 255) / 256), blk, 0, st, (const float *)t.gpart,
+                       kHid, kOut, G + kOffW3, G + kOffB3);
     hipLaunchKernelGGL(k_back_hidden<kHid>, gh, blk, 0, st, (const float *)t.dZ2, P + kOffW2,
                        (const float *)t.Z1, B, t.dZ1);
-    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1), blk, 0, st, (const float *)t.H1, kHid, seed, it, t.S,
-                       (const float *)t.dZ2, kHid, B, kHid, G + kOffW2, G + kOffB2);
-    hipLaunchKernelGGL(k_wgrad<true>, dim3(kFeat + 1), blk, 0, st, t.X, kFeat, seed, it, t.S,
-                       (const float *)t.dZ1, kHid, B, kFeat, G, G + kOffB1);
+    hipLaunchKernelGGL(k_wgrad<false>, dim3(kHid + 1, kGradChunks), blk, 0, st, (const float *)t.H1, kHid, seed, it,
+                       t.S, (const float *)t.dZ2, kHid, B, kHid, t.gpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(((kHid + 1) * kHid + 255) / 256), blk, 0, st, (const float *)t.gpart,
+                       kHid, kHid, G + kOffW2, G + kOffB2);
+    hipLaunchKernelGGL(k_wgrad<true>, dim3(kFeat + 1, kGradChunks), blk, 0, st, t.X, kFeat, seed, it, t.S,
+                       (const float *)t.dZ1, kHid, B, kFeat, t.gpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(((kFeat + 1) * kHid + 255) / 256), blk, 0, st, (const float *)t.gpart,
+                       kFeat, kHid, G, G + kOffB1);
     hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), blk, 0, st, t.P, (const float *)G, t.M, t.V,
                        kMlpParams, lr, beta1, beta2, eps, c1, c2, (const float *)t.loss_part, nparts, B,
                        t.loss_hist + it, adam);

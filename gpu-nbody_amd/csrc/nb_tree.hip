@@ -204,18 +204,10 @@ __device__ __forceinline__ void set_path_oct(unsigned long long &w0, unsigned lo
     else w2 = (w2 & clr) | val;
 }
 
-// Cells for the active lanes with one atomic per wave.
-__device__ __forceinline__ int alloc_cell(Ctl *ctl)
-{
-    const unsigned long long act = __ballot(1);
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)act) - 1;
-    const int rank = __popcll(act & ((1ull << lane) - 1ull));
-    int base = 0;
-    if (lane == leader) base = __hip_atomic_fetch_add(&ctl->n_cells, __popcll(act), __ATOMIC_RELAXED, AGENT);
-    base = __shfl(base, leader);
-    return base + rank;
-}
+// A thread takes cells from the pool kAllocChunk at a time (one atomic per
+// chunk). Unused cells of a chunk stay dead: they are never linked, so the
+// level lists never see them; the pool carries the slack.
+constexpr int kAllocChunk = 16;
 
 // seed: insert sorted[k * stride] for k < nseed; otherwise every other body,
 // a contiguous chunk of the sorted order per thread
@@ -246,6 +238,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int
     }
     int my_max = 0;
     int spare = -1;
+    int pool_next = 0, pool_end = 0; // this thread's chunk of the cell pool
     bool failed = false;
     unsigned long long po0 = 0, po1 = 0, po2 = 0; // octants of the cached path
     int plen = 0; // pc[1 .. plen-1] hold the cells of the cached path
@@ -297,7 +290,13 @@ __global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int
                     // slot holds body v: split it (buildtree.cl:102-181)
                     if (level + 1 >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
                     int c = spare;
-                    if (c < 0) c = alloc_cell(ctl);
+                    if (c < 0) {
+                        if (pool_next == pool_end) {
+                            pool_next = __hip_atomic_fetch_add(&ctl->n_cells, kAllocChunk, __ATOMIC_RELAXED, AGENT);
+                            pool_end = pool_next + kAllocChunk;
+                        }
+                        c = pool_next++;
+                    }
                     spare = -1;
                     if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
                     if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELAXED,
@@ -448,6 +447,7 @@ __global__ __launch_bounds__(256) void k_level_scatter(Dev d)
 // from its children's results of the previous launch; children are compacted
 // to the front (summarizetree.cl:75-81) into the traversal list.
 // ---------------------------------------------------------------------------
+template <bool QUAD>
 __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int cells)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,34 +462,72 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     double m = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     int cnt = 0, used = 0;
     int *out = d.cchild + (size_t)c * 8;
+    // learned mode: each child's mass and centre stay in registers for the
+    // second-moment pass, and the children's own moments are summed here
+    double cm[QUAD ? 8 : 1], cpx[QUAD ? 8 : 1], cpy[QUAD ? 8 : 1], cpz[QUAD ? 8 : 1];
+    double qs[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double eps2 = (double)d.eps2;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int v = ch[k];
+        if (QUAD) cm[k] = cpx[k] = cpy[k] = cpz[k] = 0.0;
         if (v < 0) continue;
         float4 *cd = d.cdata + (size_t)c * 8 + used;
-        out[used++] = v;
+        float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = make_float4(0.f, 0.f, 0.f, 0.f);
+        double mk, xk, yk, zk;
         if (v < n) {
             const float4 b = d.pm4[v];
             *cd = b;
-            const double mb = (double)b.w;
-            m += mb;
-            sx = fma(mb, (double)b.x, sx);
-            sy = fma(mb, (double)b.y, sy);
-            sz = fma(mb, (double)b.z, sz);
+            mk = b.w;
+            xk = b.x;
+            yk = b.y;
+            zk = b.z;
             cnt += 1;
         } else {
             const double4 q = d.com64[v - n];
             *cd = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-            m += q.w;
-            sx = fma(q.w, q.x, sx);
-            sy = fma(q.w, q.y, sy);
-            sz = fma(q.w, q.z, sz);
+            mk = q.w;
+            xk = q.x;
+            yk = q.y;
+            zk = q.z;
             cnt += d.count[v - n];
+            if (QUAD) {
+                const double4 *cq = reinterpret_cast<const double4 *>(d.cq64 + (size_t)(v - n) * 8);
+                const double4 q0 = cq[0], q1 = cq[1];
+                qs[0] += q0.x;
+                qs[1] += q0.y;
+                qs[2] += q0.z;
+                qs[3] += q0.w;
+                qs[4] += q1.x;
+                qs[5] += q1.y;
+                qs[6] += q1.z;
+                qa = make_float4((float)q0.x, (float)q0.y, (float)q0.z, (float)(q1.z * eps2));
+                qb = make_float4((float)q0.w, (float)q1.x, (float)q1.y, 0.f);
+            }
         }
+        if (QUAD) {
+            d.cqa[(size_t)c * 8 + used] = qa;
+            d.cqb[(size_t)c * 8 + used] = qb;
+        }
+        if (QUAD) {
+            cm[k] = mk;
+            cpx[k] = xk;
+            cpy[k] = yk;
+            cpz[k] = zk;
+        }
+        out[used++] = v;
+        m += mk;
+        sx = fma(mk, xk, sx);
+        sy = fma(mk, yk, sy);
+        sz = fma(mk, zk, sz);
     }
     for (int k = used; k < 8; ++k) {
         out[k] = -1;
         d.cdata[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (QUAD) {
+            d.cqa[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            d.cqb[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
     // a cell of zero mass gets its centre at the origin instead of the
     // reference's 0 * inf = NaN (summarizetree.cl:161-167)
@@ -497,50 +535,30 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
     d.com64[c] = com;
     d.count[c] = cnt;
-    if (d.quad) {
+    if (QUAD) {
         // Second moments about the centre of mass for the softened quadrupole
         // of the learned mode: the traceless tensor Q = sum m (3 s s^T - |s|^2 I)
-        // and the trace T = sum m |s|^2 (s = offset from the centre of mass),
-        // moved to the parent's centre by the parallel-axis rule. Each child's
-        // (Qxx, Qyy, Qxy, Qxz) and (Qyz, T eps^2) are also written next to its
-        // (x, y, z, m) for the traversal; bodies have no moments.
-        double q[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        const double eps2 = (double)d.eps2;
-        for (int k = 0; k < used; ++k) {
-            const int v = out[k];
-            double mx, px, py, pz;
-            float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
-            float2 qb = make_float2(0.f, 0.f);
-            if (v < n) {
-                const float4 b = d.pm4[v];
-                mx = b.w; px = b.x; py = b.y; pz = b.z;
-            } else {
-                const double4 cc = d.com64[v - n];
-                mx = cc.w; px = cc.x; py = cc.y; pz = cc.z;
-                const double *cq = d.cq64 + (size_t)(v - n) * 8;
-                for (int j = 0; j < 7; ++j) q[j] += cq[j];
-                qa = make_float4((float)cq[0], (float)cq[1], (float)cq[3], (float)cq[4]);
-                qb = make_float2((float)cq[5], (float)(cq[6] * eps2));
-            }
-            d.cqa[(size_t)c * 8 + k] = qa;
-            d.cqb[(size_t)c * 8 + k] = qb;
-            const double dx = px - com.x, dy = py - com.y, dz = pz - com.z;
+        // and the trace T = sum m |s|^2 (s = offset from the centre of mass):
+        // the children's own moments (summed above) plus, by the parallel-axis
+        // rule, each child's mass at its offset from this cell's centre of mass.
+        // Each child's moments also sit next to its (x, y, z, m) for the
+        // traversal, as (Qxx, Qyy, Qzz, T eps^2) and (Qxy, Qxz, Qyz, 0).
+#pragma unroll
+        for (int k = 0; k < (QUAD ? 8 : 0); ++k) {
+            const double mx = cm[k];
+            const double dx = cpx[k] - com.x, dy = cpy[k] - com.y, dz = cpz[k] - com.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            q[0] += mx * (3.0 * dx * dx - r2);
-            q[1] += mx * (3.0 * dy * dy - r2);
-            q[2] += mx * (3.0 * dz * dz - r2);
-            q[3] += mx * 3.0 * dx * dy;
-            q[4] += mx * 3.0 * dx * dz;
-            q[5] += mx * 3.0 * dy * dz;
-            q[6] += mx * r2;
+            qs[0] += mx * (3.0 * dx * dx - r2);
+            qs[1] += mx * (3.0 * dy * dy - r2);
+            qs[2] += mx * (3.0 * dz * dz - r2);
+            qs[3] += mx * 3.0 * dx * dy;
+            qs[4] += mx * 3.0 * dx * dz;
+            qs[5] += mx * 3.0 * dy * dz;
+            qs[6] += mx * r2;
         }
-        for (int k = used; k < 8; ++k) {
-            d.cqa[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            d.cqb[(size_t)c * 8 + k] = make_float2(0.f, 0.f);
-        }
-        double *cq = d.cq64 + (size_t)c * 8;
-        for (int j = 0; j < 7; ++j) cq[j] = q[j];
-        cq[7] = 0.0;
+        double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);
+        cq[0] = make_double4(qs[0], qs[1], qs[2], qs[3]);
+        cq[1] = make_double4(qs[4], qs[5], qs[6], 0.0);
     }
 }
 
@@ -679,7 +697,10 @@ hipError_t launch_levels(const Dev &d, hipStream_t st)
 hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st)
 {
     if (cells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_summarize_level, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+    if (d.quad)
+        hipLaunchKernelGGL(k_summarize_level<true>, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+    else
+        hipLaunchKernelGGL(k_summarize_level<false>, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
     return hipGetLastError();
 }
 hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)

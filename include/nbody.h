@@ -194,7 +194,9 @@ int nb_force_error(nb_sim *sim, int32_t nsamples, uint64_t seed, double *stats4)
  * W1[24][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3] (W[i][j] maps input
  * i to output j), then the feature means mu_x[24] and standard deviations
  * sd_x[24], then the target means mu_y[3] and standard deviations sd_y[3]:
- * residual = (MLP((x - mu_x) / sd_x)) * sd_y + mu_y. */
+ * residual = MLP(clip((x - mu_x) / sd_x)) * sd_y + mu_y, where clip limits each
+ * normalized feature to [-8, 8] (training clips its normalized targets the same
+ * way). */
 #define NB_MLP_IN 24
 #define NB_MLP_HIDDEN 64
 #define NB_MLP_OUT 3

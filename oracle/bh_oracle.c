@@ -841,6 +841,12 @@ void orc_moments(orc_sim *s)
     for (int node = s->bottom; node <= nn; ++node) {
         const double *cc = s->com + 4 * (size_t)node;
         double q[7] = {0, 0, 0, 0, 0, 0, 0};
+        /* the children's own moments, then each child's mass at its offset */
+        for (int k = 0; k < 8; ++k) {
+            const int c = s->child[8 * node + k];
+            if (c >= n)
+                for (int j = 0; j < 7; ++j) q[j] += s->cq[8 * (size_t)c + j];
+        }
         for (int k = 0; k < 8; ++k) {
             const int c = s->child[8 * node + k];
             if (c < 0) continue;
@@ -848,7 +854,6 @@ void orc_moments(orc_sim *s)
             if (c >= n) {
                 const double *c2 = s->com + 4 * (size_t)c;
                 mx = c2[3]; px = c2[0]; py = c2[1]; pz = c2[2];
-                for (int j = 0; j < 7; ++j) q[j] += s->cq[8 * (size_t)c + j];
             } else {
                 mx = s->m[c]; px = s->x[c]; py = s->y[c]; pz = s->z[c];
             }

@@ -108,7 +108,7 @@ def test_mlp_bf16_inference_vs_oracle(nb, oracle):
     for rows in (1000, 4096, 77):
         x = (xmu + xsd * rng.normal(size=(rows, 24))).astype(np.float32)
         y = sim.mlp_forward(x)
-        xn = ((x - xmu) * (np.float32(1) / xsd)).astype(np.float32)
+        xn = np.clip((x - xmu) * (np.float32(1) / xsd), -8, 8).astype(np.float32)
         ref = oracle.mlp_forward(blob[:5955].astype(np.float64), xn) * ysd + ymu
         d = y - ref
         assert np.sqrt((d ** 2).mean()) <= 0.01 * np.sqrt((ref ** 2).mean()), (rows, np.sqrt((d ** 2).mean()))
