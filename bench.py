@@ -104,7 +104,7 @@ def stage_profile(sim, steps):
     sim.steps(steps)
     sim.set_timing(False)
     c = sim.counters()
-    names = ("bbox", "build", "summarize", "sort", "force", "integrate", "network")
+    names = ("bbox", "build", "summarize", "sort", "force", "integrate")
     return {k: round(c.stage_ms[i], 4) for i, k in enumerate(names)}, c.interactions / steps
 
 
@@ -201,7 +201,7 @@ def main():
         sim.close()
 
     main_r = results[mode]
-    force_ms = main_r["stages"]["force"]  # the traversal alone (the network is its own stage)
+    force_ms = main_r["stages"]["force"]  # the traversal (and, learned, the network and the body update)
     achieved = FLOP_PER_INTERACTION * main_r["interactions"] / (force_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "force_traffic.json")

@@ -49,7 +49,6 @@ struct nb_sim {
     long long steps_since_model = 0;
     uint64_t check_seed = 0x5eed;
     float4 *chk_acc = nullptr;
-    hipEvent_t mid_ev = nullptr; // timing: between the coarse traversal and the network
     std::string err;
 };
 
@@ -196,10 +195,8 @@ int launch_force_method(nb_sim *s, int method, int fmode)
         break;
     case NB_MODE_LEARNED:
         if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
-        e = nb::launch_coarse(d, s->L, nb::kOutInfer, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
+        e = nb::launch_coarse(d, s->L, nb::kOutLearned, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
                               fmode, s->stream);
-        if (e == hipSuccess && s->timing && s->mid_ev) e = hipEventRecord(s->mid_ev, s->stream);
-        if (e == hipSuccess) e = nb::launch_mlp(d, s->L, s->L.feat, d.n, 0, nullptr, fmode, s->stream);
         break;
     default: return fail(s, NB_E_ARG, "unknown force method %d", method);
     }
@@ -252,9 +249,6 @@ int ensure_quad(nb_sim *s)
 
 int ensure_learn(nb_sim *s)
 {
-    const size_t n = (size_t)s->d.n;
-    if (!s->L.feat) HIP_TRY(s, dalloc(&s->L.feat, n * nb::kFeat));
-    if (!s->L.frame) HIP_TRY(s, dalloc(&s->L.frame, 2 * n));
     if (!s->mlpbuf) HIP_TRY(s, hipMalloc(&s->mlpbuf, nb::kMlpBufBytes));
     return ensure_quad(s);
 }
@@ -282,17 +276,33 @@ int install_model(nb_sim *s, const float *blob)
     int r = ensure_learn(s);
     if (r) return r;
     std::vector<unsigned char> buf(nb::kMlpBufBytes, 0);
-    uint16_t *w1t = reinterpret_cast<uint16_t *>(buf.data());
-    uint16_t *w2t = w1t + 64 * 32;
-    uint16_t *w3t = w2t + 64 * 64;
-    float *fb = reinterpret_cast<float *>(w3t + 16 * 64);
+    uint16_t *w1f = reinterpret_cast<uint16_t *>(buf.data());
+    uint16_t *w2f = w1f + nb::kW1Frag;
+    uint16_t *w3f = w2f + nb::kW2Frag;
+    float *fb = reinterpret_cast<float *>(w3f + nb::kW3Frag);
     float *b1 = fb, *b2 = fb + 64, *b3 = fb + 128, *mu = fb + 144, *rs = fb + 168;
-    for (int nn = 0; nn < 64; ++nn)
-        for (int k = 0; k < nb::kFeat; ++k) w1t[nn * 32 + k] = bf16_rne(P[k * 64 + nn]);
-    for (int nn = 0; nn < 64; ++nn)
-        for (int k = 0; k < 64; ++k) w2t[nn * 64 + k] = bf16_rne(P[nb::kOffW2 + k * 64 + nn]);
-    for (int nn = 0; nn < nb::kOut; ++nn)
-        for (int k = 0; k < 64; ++k) w3t[nn * 64 + k] = bf16_rne(P[nb::kOffW3 + k * 3 + nn] * ysd[nn]);
+    // fragment order of the transposed network (nb_internal.h, struct Mlp):
+    // lane l holds row l % 16 of its 16-row block and k group g = l / 16
+    auto pk = [](int kb, int g, int i) { return 32 * kb + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4)); };
+    for (int rb = 0; rb < 4; ++rb)
+        for (int l = 0; l < 64; ++l)
+            for (int i = 0; i < 8; ++i) {
+                const int k = 8 * (l / 16) + i, nn = 16 * rb + l % 16;
+                w1f[(rb * 64 + l) * 8 + i] = k < nb::kFeat ? bf16_rne(P[k * 64 + nn]) : 0;
+            }
+    for (int rb = 0; rb < 4; ++rb)
+        for (int kb = 0; kb < 2; ++kb)
+            for (int l = 0; l < 64; ++l)
+                for (int i = 0; i < 8; ++i) {
+                    const int k = pk(kb, l / 16, i), nn = 16 * rb + l % 16;
+                    w2f[((rb * 2 + kb) * 64 + l) * 8 + i] = bf16_rne(P[nb::kOffW2 + k * 64 + nn]);
+                }
+    for (int kb = 0; kb < 2; ++kb)
+        for (int l = 0; l < 64; ++l)
+            for (int i = 0; i < 8; ++i) {
+                const int k = pk(kb, l / 16, i), o = l % 16;
+                w3f[(kb * 64 + l) * 8 + i] = o < nb::kOut ? bf16_rne(P[nb::kOffW3 + k * 3 + o] * ysd[o]) : 0;
+            }
     for (int i = 0; i < 64; ++i) {
         b1[i] = P[nb::kOffB1 + i];
         b2[i] = P[nb::kOffB2 + i];
@@ -305,9 +315,9 @@ int install_model(nb_sim *s, const float *blob)
     HIP_TRY(s, hipMemcpy(s->mlpbuf, buf.data(), buf.size(), hipMemcpyHostToDevice));
     const unsigned char *base = static_cast<const unsigned char *>(s->mlpbuf);
     const size_t fo = (size_t)(reinterpret_cast<unsigned char *>(fb) - buf.data());
-    s->L.mlp.w1t = reinterpret_cast<const uint16_t *>(base);
-    s->L.mlp.w2t = reinterpret_cast<const uint16_t *>(base) + 64 * 32;
-    s->L.mlp.w3t = reinterpret_cast<const uint16_t *>(base) + 64 * 32 + 64 * 64;
+    s->L.mlp.w1f = reinterpret_cast<const uint16_t *>(base);
+    s->L.mlp.w2f = s->L.mlp.w1f + nb::kW1Frag;
+    s->L.mlp.w3f = s->L.mlp.w2f + nb::kW2Frag;
     const float *dfb = reinterpret_cast<const float *>(base + fo);
     s->L.mlp.b1 = dfb;
     s->L.mlp.b2 = dfb + 64;
@@ -328,12 +338,11 @@ void free_all(nb_sim *s)
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb,
-                    s->L.feat, s->L.frame, s->mlpbuf, s->chk_acc};
+                    s->mlpbuf, s->chk_acc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
         if (e) (void)hipEventDestroy(e);
-    if (s->mid_ev) (void)hipEventDestroy(s->mid_ev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -626,7 +635,6 @@ int nb_step(nb_sim *s, int32_t nsteps)
         HIP_TRY(s, hipEventCreate(&e));
         s->tev.push_back(e);
     }
-    std::vector<hipEvent_t> mids;
     const bool learned = s->p.mode == NB_MODE_LEARNED;
     for (int k = 0; k < nsteps; ++k) {
         hipEvent_t *ev = s->timing ? s->tev.data() + 7 * (size_t)k : nullptr;
@@ -635,14 +643,6 @@ int nb_step(nb_sim *s, int32_t nsteps)
         // back to plain theta if the median error is above check_tol
         const bool check = learned && !s->fell_back && s->p.check_every > 0 && s->p.check_tol > 0.0f &&
                            s->p.check_samples > 0 && (s->steps_since_model % s->p.check_every) == 0;
-        if (ev && force_method(s) == NB_MODE_LEARNED) {
-            hipEvent_t m;
-            HIP_TRY(s, hipEventCreate(&m));
-            mids.push_back(m);
-            s->mid_ev = m;
-        } else {
-            s->mid_ev = nullptr;
-        }
         for (int st = NB_STAGE_BBOX; st <= NB_STAGE_INTEGRATE; ++st) {
             if (ev) HIP_TRY(s, hipEventRecord(ev[st - 1], s->stream));
             int r;
@@ -658,7 +658,6 @@ int nb_step(nb_sim *s, int32_t nsteps)
         if (ev) HIP_TRY(s, hipEventRecord(ev[6], s->stream));
         if (learned) ++s->steps_since_model;
     }
-    s->mid_ev = nullptr;
     int r = check_device_error(s);
     if (s->timing && nsteps > 0) {
         for (int st = 0; st < 6; ++st) {
@@ -670,19 +669,7 @@ int nb_step(nb_sim *s, int32_t nsteps)
             }
             s->stage_ms[st] = tot / nsteps;
         }
-        // learned mode: stage_ms[6] is the network, split off the force stage
-        double net = 0.0;
-        if (mids.size() == (size_t)nsteps) {
-            for (int k = 0; k < nsteps; ++k) {
-                float ms = 0.f;
-                if (hipEventElapsedTime(&ms, mids[k], s->tev[7 * (size_t)k + 5]) == hipSuccess) net += ms;
-            }
-            net /= nsteps;
-        }
-        s->stage_ms[6] = net;
-        s->stage_ms[4] -= net;
     }
-    for (hipEvent_t m : mids) (void)hipEventDestroy(m);
     return r;
 }
 
@@ -1190,14 +1177,11 @@ int nb_mlp_forward(nb_sim *s, const float *x, int32_t n, float *y)
     HIP_TRY(s, hipSetDevice(s->device));
     Scratch m(s);
     float *xd = nullptr, *yd = nullptr;
-    uint16_t *fb = nullptr;
     int r = m.get(&xd, (size_t)n * nb::kFeat);
-    if (!r) r = m.get(&fb, (size_t)n * nb::kFeat);
     if (!r) r = m.get(&yd, (size_t)n * nb::kOut);
     if (r) return r;
     HIP_TRY(s, hipMemcpyAsync(xd, x, (size_t)n * nb::kFeat * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    hipError_t e = nb::launch_norm_bf16(s->L, xd, n, fb, s->stream);
-    if (e == hipSuccess) e = nb::launch_mlp(s->d, s->L, fb, n, 1, yd, 0, s->stream);
+    hipError_t e = nb::launch_mlp_rows(s->L, xd, n, yd, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "network launch: %s", hipGetErrorString(e));
     HIP_TRY(s, hipMemcpyAsync(y, yd, (size_t)n * nb::kOut * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));

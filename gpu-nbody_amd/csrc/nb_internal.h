@@ -124,43 +124,47 @@ constexpr int kMlpParams = kFeat * kHid + kHid + kHid * kHid + kHid + kHid * kOu
 constexpr int kOffB1 = kFeat * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid;
 constexpr int kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kOut;
 
-// Inference network for the matrix cores: BF16 weights stored per output
-// neuron (w1t[64][32], inputs 24..31 zero; w2t[64][64]; w3t[16][64], outputs
-// 3..15 zero, target scale folded in), FP32 biases (b3 has 16 entries with the
-// target mean folded in) and the feature normalization (mean, 1/sd).
+// Inference network for the matrix cores, in the transposed form
+// Z^T = W^T X^T (bodies are the columns), stored as the per-lane A-operand
+// fragments of v_mfma_f32_16x16x32_bf16 (8 BF16 per lane, lane l: row l % 16
+// of the 16-row block, k group g = l / 16):
+//   w1f[rb][lane][i]      W1[8 g + i][16 rb + l % 16]            (inputs 24..31: 0)
+//   w2f[rb][kb][lane][i]  W2[pk(kb, g, i)][16 rb + l % 16]
+//   w3f[kb][lane][i]      W3[pk(kb, g, i)][l % 16] * sd_y         (outputs 3..15: 0)
+// where pk(kb, g, i) = 32 kb + 4 g + i for i < 4 and 32 kb + 16 + 4 g + i - 4
+// for i >= 4 is the order in which a lane already holds the previous layer's
+// outputs, so activations pass from layer to layer without moving between
+// lanes. FP32 biases (b3 with the target mean folded in, 16 entries) and the
+// feature normalization (mean, 1/sd) follow.
 struct Mlp {
-    const uint16_t *w1t, *w2t, *w3t;
+    const uint16_t *w1f, *w2f, *w3f;
     const float *b1, *b2, *b3;
     const float *xmu, *xrs;
 };
+constexpr int kW1Frag = 4 * 64 * 8, kW2Frag = 4 * 2 * 64 * 8, kW3Frag = 2 * 64 * 8; // bf16 counts
 // normalized features and training targets are clipped to [-kClipZ, kClipZ]:
 // both have rare tails of tens of standard deviations (bodies whose coarse
 // acceleration nearly cancels), which would otherwise dominate the squared loss
 constexpr float kClipZ = 8.0f;
 constexpr int kGradChunks = 16; // row chunks of the weight-gradient reduction
-constexpr size_t kMlpBufBytes = (64 * 32 + 64 * 64 + 16 * 64) * 2 + (64 + 64 + 16 + 24 + 24) * 4;
+constexpr size_t kMlpBufBytes = (size_t)(kW1Frag + kW2Frag + kW3Frag) * 2 + (64 + 64 + 16 + 24 + 24) * 4;
 
 struct Learn {
-    uint16_t *feat; // bf16 [N][24] normalized features, by sorted position
-    float4 *frame;  // [2N] by sorted position: (a, |a|) and (|a| e2, 0)
     Mlp mlp;
 };
 
-// What the coarse traversal writes: normalized BF16 features and frames for
-// every body (inference), raw FP32 features and frames for listed groups
-// (training pairs), accelerations of listed groups (training targets), or the
-// finished body update (the coarse method alone).
-enum CoarseOut { kOutInfer = 0, kOutRaw = 1, kOutAccel = 2, kOutFinish = 3 };
+// What the coarse traversal does with its result: the learned step (features
+// through the network, corrected acceleration, body update), raw FP32
+// features and frames of listed groups (training pairs), accelerations of
+// listed groups (training targets), or the body update with the coarse
+// acceleration alone.
+enum CoarseOut { kOutLearned = 0, kOutRaw = 1, kOutAccel = 2, kOutFinish = 3 };
 hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,
                          int ngroups, float *xraw, float4 *frame_out, float4 *acc_out, int fmode,
                          hipStream_t st);
-// BF16 matrix-core inference over rows [0, rows) of feat: sink 0 applies the
-// correction and finishes the bodies (fmode as launch_force), sink 1 writes
-// the raw residual y[rows][3]
-hipError_t launch_mlp(const Dev &d, const Learn &L, const uint16_t *feat, int rows, int sink, float *y,
-                      int fmode, hipStream_t st);
-// raw features -> normalized BF16 (the inference input format)
-hipError_t launch_norm_bf16(const Learn &L, const float *xraw, int rows, uint16_t *out, hipStream_t st);
+// The network alone on raw FP32 features x[rows][24] -> residual y[rows][3]
+// (the same matrix-core code as the learned step)
+hipError_t launch_mlp_rows(const Learn &L, const float *xraw, int rows, float *y, hipStream_t st);
 // training targets (a_fine - a) in the frame over |a|
 hipError_t launch_make_target(const float4 *frame, const float4 *afine, int rows, float *t, hipStream_t st);
 // per-column mean and standard deviation (sd 1 for a constant column), then

@@ -30,11 +30,18 @@
 //   22    log(group radius / root radius)
 //   23    cosine between the cell monopole part and a_b
 //
-// Network. 24 -> 64 -> 64 -> 3 with tanh-GELU. Inference: one wave runs its
-// 64 bodies through v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32
-// accumulation): 16 + 32 + 8 matrix instructions per wave; activations pass
-// between layers through LDS. Training: FP32 kernels for the forward pass,
-// the backward pass and Adam, with deterministic reductions.
+// Network. 24 -> 64 -> 64 -> 3 with tanh-GELU. Inference runs in the epilogue
+// of the coarse traversal itself: the wave's 64 bodies go through
+// v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation) in the
+// transposed form Z^T = W^T X^T, 16 bodies (one B-operand column block) at a
+// time: 4 + 8 + 2 matrix instructions per block. The features reach the
+// operand layout through LDS once; between layers, each lane's D registers
+// already hold what it needs as the next B operand (the weights are stored
+// with the matching order of k), and the three outputs return to their own
+// lanes by shuffles. The corrected acceleration then goes straight into the
+// body update, so the update overlaps with other waves' traversal. Training:
+// FP32 kernels for the forward pass, the backward pass and Adam, with
+// deterministic reductions.
 #include "nb_device.h"
 
 namespace nb {
@@ -83,11 +90,83 @@ __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t *p)
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(p));
 }
 
+constexpr int kStackC = 768; // >= kStack; 768 words = 64 rows of 24 BF16 features
+static_assert(kStackC >= kStack && kStackC * 4 >= 64 * kFeat * 2, "coarse stack doubles as the feature rows");
+
+// ---------------------------------------------------------------------------
+// BF16 matrix-core network (shared by the learned step and the test path)
+//
+// v_mfma_f32_16x16x32_bf16 operand layout (lane l, i16 = l % 16, g = l / 16):
+//   A: row i16 of the 16-row block, k = 8 g .. 8 g + 7
+//   B: column i16 of the 16-column block, k = 8 g .. 8 g + 7
+//   D: column i16, rows 4 g .. 4 g + 3
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bf16x8 pack_gelu(f32x4 lo, float4 blo, f32x4 hi, float4 bhi)
+{
+    u16x8 h;
+    h[0] = to_bf16(gelu(lo[0] + blo.x));
+    h[1] = to_bf16(gelu(lo[1] + blo.y));
+    h[2] = to_bf16(gelu(lo[2] + blo.z));
+    h[3] = to_bf16(gelu(lo[3] + blo.w));
+    h[4] = to_bf16(gelu(hi[0] + bhi.x));
+    h[5] = to_bf16(gelu(hi[1] + bhi.y));
+    h[6] = to_bf16(gelu(hi[2] + bhi.z));
+    h[7] = to_bf16(gelu(hi[3] + bhi.w));
+    return __builtin_bit_cast(bf16x8, h);
+}
+
+// Runs the wave's 64 rows of normalized BF16 features (fs: LDS, row r at
+// fs + 24 r) through the network; lane r receives the residual of row r.
+__device__ __forceinline__ void mlp_wave(const uint16_t *fs, const Mlp &m, int lane, float &y0, float &y1,
+                                         float &y2)
+{
+    const int i16 = lane & 15, g = lane >> 4;
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+    y0 = y1 = y2 = 0.f;
+    // one 16-body column block at a time, not unrolled: the epilogue must not
+    // raise the register count of the traversal it ends
+#pragma unroll 1
+    for (int cb = 0; cb < 4; ++cb) {
+        const bf16x8 xb = g < 3 ? load_bf16x8(fs + (cb * 16 + i16) * kFeat + 8 * g)
+                                : __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
+        f32x4 d1[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+            d1[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w1f + (rb * 64 + lane) * 8), xb, z4, 0, 0, 0);
+        const float4 *b1 = reinterpret_cast<const float4 *>(m.b1 + 4 * g);
+        const bf16x8 h1a = pack_gelu(d1[0], b1[0], d1[1], b1[4]);
+        const bf16x8 h1b = pack_gelu(d1[2], b1[8], d1[3], b1[12]);
+        // keep each layer's weight loads next to their use (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+        f32x4 d2[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            d2[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 0) * 64 + lane) * 8), h1a,
+                                                             z4, 0, 0, 0);
+            d2[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 1) * 64 + lane) * 8), h1b,
+                                                             d2[rb], 0, 0, 0);
+        }
+        const float4 *b2 = reinterpret_cast<const float4 *>(m.b2 + 4 * g);
+        const bf16x8 h2a = pack_gelu(d2[0], b2[0], d2[1], b2[4]);
+        const bf16x8 h2b = pack_gelu(d2[2], b2[8], d2[3], b2[12]);
+        __builtin_amdgcn_sched_barrier(0);
+        f32x4 d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w3f + lane * 8), h2a, z4, 0, 0, 0);
+        d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w3f + (64 + lane) * 8), h2b, d3, 0, 0, 0);
+        // outputs 0..2 of the column block's bodies sit in lanes 0..15 (g = 0)
+        const float t0 = __shfl(d3[0], i16), t1 = __shfl(d3[1], i16), t2 = __shfl(d3[2], i16);
+        if (g == cb) {
+            y0 = t0 + m.b3[0];
+            y1 = t1 + m.b3[1];
+            y2 = t2 + m.b3[2];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Coarse traversal
 // ---------------------------------------------------------------------------
 template <int OUT>
-__global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
                                                   const float4 *__restrict__ cdata,
                                                   const float4 *__restrict__ cqa,
                                                   const float4 *__restrict__ cqb,
@@ -96,8 +175,10 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                                                   float *__restrict__ xraw, float4 *__restrict__ frame_out,
                                                   float4 *__restrict__ acc_out, int fmode)
 {
-    constexpr bool kFeatOn = OUT == kOutInfer || OUT == kOutRaw;
-    __shared__ unsigned stk[4][kStack];
+    constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
+    // a wave's traversal stack; after the traversal the learned step reuses it
+    // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
+    __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
     __shared__ float dq_s[kMaxDepth];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -221,7 +302,7 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
                 }
             } else {
                 // the tree is at most kMaxDepth deep: anything else is corrupt
-                if (sp >= kStack || v - n >= cap || dep + 1 >= kMaxDepth) {
+                if (sp >= kStackC || v - n >= cap || dep + 1 >= kMaxDepth) {
                     bad = true;
                     break;
                 }
@@ -328,31 +409,44 @@ __global__ __launch_bounds__(256) void k_coarse64(Dev d, Learn L, const int *__r
             f[23] = (mn > 0.f && bn > 0.f) ? (mx * bx + my * by + mz * bz) / (mn * bn) : 0.f;
 #undef NB_LOC
         }
-        if (valid) {
-            const float4 fr0 = make_float4(ax, ay, az, A);
-            const float4 fr1 = make_float4(A * e2x, A * e2y, A * e2z, 0.f);
-            if constexpr (OUT == kOutRaw) {
+        if constexpr (OUT == kOutRaw) {
+            if (valid) {
                 float *o = xraw + (size_t)slot * kFeat;
 #pragma unroll
                 for (int i = 0; i < kFeat; i += 4)
                     *reinterpret_cast<float4 *>(o + i) = make_float4(f[i], f[i + 1], f[i + 2], f[i + 3]);
-                frame_out[2 * (size_t)slot] = fr0;
-                frame_out[2 * (size_t)slot + 1] = fr1;
-            } else {
-                uint32_t w[kFeat / 2];
-#pragma unroll
-                for (int i = 0; i < kFeat; i += 2) {
-                    const uint32_t lo = to_bf16(clipz((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]));
-                    const uint32_t hi = to_bf16(clipz((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]));
-                    w[i / 2] = lo | (hi << 16);
-                }
-                uint4 *o = reinterpret_cast<uint4 *>(L.feat + (size_t)pos * kFeat);
-                o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-                o[1] = make_uint4(w[4], w[5], w[6], w[7]);
-                o[2] = make_uint4(w[8], w[9], w[10], w[11]);
-                L.frame[2 * (size_t)pos] = fr0;
-                L.frame[2 * (size_t)pos + 1] = fr1;
+                frame_out[2 * (size_t)slot] = make_float4(ax, ay, az, A);
+                frame_out[2 * (size_t)slot + 1] = make_float4(A * e2x, A * e2y, A * e2z, 0.f);
             }
+        } else {
+            // the learned step: normalized BF16 features into this wave's LDS
+            // rows, the network, the correction, the body update
+            uint16_t *fs = reinterpret_cast<uint16_t *>(stk[wv]);
+            uint32_t w[kFeat / 2];
+#pragma unroll
+            for (int i = 0; i < kFeat; i += 2) {
+                const uint32_t lo = to_bf16(clipz((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]));
+                const uint32_t hi = to_bf16(clipz((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]));
+                w[i / 2] = valid ? (lo | (hi << 16)) : 0u;
+            }
+            uint4 *o = reinterpret_cast<uint4 *>(fs + lane * kFeat);
+            o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+            o[2] = make_uint4(w[8], w[9], w[10], w[11]);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            float y0, y1, y2;
+            mlp_wave(fs, L.mlp, lane, y0, y1, y2);
+            // residual = |a| (y0 e1 + y1 e2 + y2 e3)
+            float rx = ax, ry = ay, rz = az;
+            if (A > 0.f) {
+                const float e1x = ax / A, e1y = ay / A, e1z = az / A;
+                const float e3x = e1y * e2z - e1z * e2y, e3y = e1z * e2x - e1x * e2z, e3z = e1x * e2y - e1y * e2x;
+                rx += A * (y0 * e1x + y1 * e2x + y2 * e3x);
+                ry += A * (y0 * e1y + y1 * e2y + y2 * e3y);
+                rz += A * (y0 * e1z + y1 * e2z + y2 * e3z);
+            }
+            if (valid) finish_body(d, pos, me, rx, ry, rz, fmode, step);
         }
     }
     // device interaction counter: one atomic per wave
@@ -373,7 +467,7 @@ hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqt
     d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float4 *)d.cqb, \
         dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
     switch (out) {
-    case kOutInfer: hipLaunchKernelGGL(k_coarse64<kOutInfer>, grid, block, 0, st, NB_COARSE_ARGS); break;
+    case kOutLearned: hipLaunchKernelGGL(k_coarse64<kOutLearned>, grid, block, 0, st, NB_COARSE_ARGS); break;
     case kOutRaw: hipLaunchKernelGGL(k_coarse64<kOutRaw>, grid, block, 0, st, NB_COARSE_ARGS); break;
     case kOutAccel: hipLaunchKernelGGL(k_coarse64<kOutAccel>, grid, block, 0, st, NB_COARSE_ARGS); break;
     case kOutFinish: hipLaunchKernelGGL(k_coarse64<kOutFinish>, grid, block, 0, st, NB_COARSE_ARGS); break;
@@ -383,147 +477,31 @@ hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqt
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// BF16 matrix-core inference
-//
-// v_mfma_f32_16x16x32_bf16 operand layout (lane l, i16 = l % 16, grp = l / 16):
-//   A: row i16 of the 16-row block, k = 8 grp .. 8 grp + 7
-//   B: column i16 of the 16-column block, k = 8 grp .. 8 grp + 7
-//   D: column i16, rows 4 grp .. 4 grp + 3
-// Weights are stored per output neuron (w_t[n][k]) so that a B operand is one
-// 16-byte load; the activations of a wave's 64 bodies go through LDS as
-// row-major BF16 with a padded row stride.
-// ---------------------------------------------------------------------------
-constexpr int kHS = 72; // LDS row stride of the activations, in bf16 (144 bytes)
-
-template <int SINK>
-__global__ __launch_bounds__(256) void k_mlp(Dev d, Learn L, const uint16_t *__restrict__ feat, int rows,
-                                             float *__restrict__ yout, int fmode)
+// Test path: raw features -> normalized BF16 rows in LDS -> network -> y.
+__global__ __launch_bounds__(256) void k_mlp_rows(Mlp m, const float *__restrict__ x, int rows,
+                                                  float *__restrict__ y)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t hs[4][64 * kHS];
-    __shared__ float ys[4][64 * 4];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = tid >> 6;
-    const int base = (blockIdx.x * 4 + wv) * 64;
-    const int i16 = lane & 15, grp = lane >> 4;
-    uint16_t *h = hs[wv];
-    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
-
-    // layer 1: [64 x 32] x [32 x 64]
-    bf16x8 a[4];
+    __shared__ __attribute__((aligned(16))) uint16_t fs[4][64 * kFeat];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int row = (blockIdx.x * 4 + wv) * 64 + lane;
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-        const int row = base + rb * 16 + i16;
-        if (grp < 3 && row < rows)
-            a[rb] = load_bf16x8(feat + (size_t)row * kFeat + grp * 8);
-        else
-            a[rb] = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
-    }
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-        const bf16x8 b = load_bf16x8(L.mlp.w1t + (cb * 16 + i16) * 32 + grp * 8);
-        const float bias = L.mlp.b1[cb * 16 + i16];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb], b, zero4, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) h[(rb * 16 + grp * 4 + j) * kHS + cb * 16 + i16] = to_bf16(gelu(z[j] + bias));
-        }
-    }
+    for (int i = 0; i < kFeat; ++i)
+        fs[wv][lane * kFeat + i] = row < rows ? to_bf16(clipz((x[(size_t)row * kFeat + i] - m.xmu[i]) * m.xrs[i])) : 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
-
-    // layer 2: [64 x 64] x [64 x 64]; all A operands are read before H2 overwrites H1
-    bf16x8 a2[2][4];
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) a2[kb][rb] = load_bf16x8(h + (rb * 16 + i16) * kHS + kb * 32 + grp * 8);
-    __syncthreads();
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-        const bf16x8 b0 = load_bf16x8(L.mlp.w2t + (cb * 16 + i16) * 64 + grp * 8);
-        const bf16x8 b1 = load_bf16x8(L.mlp.w2t + (cb * 16 + i16) * 64 + 32 + grp * 8);
-        const float bias = L.mlp.b2[cb * 16 + i16];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[0][rb], b0, zero4, 0, 0, 0);
-            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[1][rb], b1, z, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) h[(rb * 16 + grp * 4 + j) * kHS + cb * 16 + i16] = to_bf16(gelu(z[j] + bias));
-        }
-    }
-    __syncthreads();
-
-    // layer 3: [64 x 64] x [64 x 16] (outputs 3..15 are zero columns)
-    {
-        const bf16x8 b0 = load_bf16x8(L.mlp.w3t + i16 * 64 + grp * 8);
-        const bf16x8 b1 = load_bf16x8(L.mlp.w3t + i16 * 64 + 32 + grp * 8);
-        const float bias = L.mlp.b3[i16];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const bf16x8 x0 = load_bf16x8(h + (rb * 16 + i16) * kHS + grp * 8);
-            const bf16x8 x1 = load_bf16x8(h + (rb * 16 + i16) * kHS + 32 + grp * 8);
-            f32x4 z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0, b0, zero4, 0, 0, 0);
-            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1, b1, z, 0, 0, 0);
-            if (i16 < 4) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) ys[wv][(rb * 16 + grp * 4 + j) * 4 + i16] = z[j] + bias;
-            }
-        }
-    }
-    __syncthreads();
-
-    const int row = base + lane;
-    if (row >= rows) return;
-    const float y0 = ys[wv][lane * 4 + 0], y1 = ys[wv][lane * 4 + 1], y2 = ys[wv][lane * 4 + 2];
-    if constexpr (SINK == 1) {
-        yout[(size_t)row * 3 + 0] = y0;
-        yout[(size_t)row * 3 + 1] = y1;
-        yout[(size_t)row * 3 + 2] = y2;
-    } else {
-        // residual = |a| (y0 e1 + y1 e2 + y2 e3) = y0 a + y1 (|a| e2) + y2 (a x |a| e2) / |a|
-        const float4 f0 = L.frame[2 * (size_t)row];
-        const float4 f1 = L.frame[2 * (size_t)row + 1];
-        float ax = f0.x, ay = f0.y, az = f0.z;
-        if (f0.w > 0.f) {
-            const float inv = 1.0f / f0.w;
-            const float cx = (f0.y * f1.z - f0.z * f1.y) * inv;
-            const float cy = (f0.z * f1.x - f0.x * f1.z) * inv;
-            const float cz = (f0.x * f1.y - f0.y * f1.x) * inv;
-            ax += y0 * f0.x + y1 * f1.x + y2 * cx;
-            ay += y0 * f0.y + y1 * f1.y + y2 * cy;
-            az += y0 * f0.z + y1 * f1.z + y2 * cz;
-        }
-        finish_body(d, row, d.bnode[row], ax, ay, az, fmode, d.ctl->step);
+    float y0, y1, y2;
+    mlp_wave(fs[wv], m, lane, y0, y1, y2);
+    if (row < rows) {
+        y[(size_t)row * 3 + 0] = y0;
+        y[(size_t)row * 3 + 1] = y1;
+        y[(size_t)row * 3 + 2] = y2;
     }
 }
 
-hipError_t launch_mlp(const Dev &d, const Learn &L, const uint16_t *feat, int rows, int sink, float *y,
-                      int fmode, hipStream_t st)
+hipError_t launch_mlp_rows(const Learn &L, const float *xraw, int rows, float *y, hipStream_t st)
 {
     if (rows <= 0) return hipSuccess;
-    const dim3 grid((rows + 255) / 256), block(256);
-    if (sink == 0)
-        hipLaunchKernelGGL(k_mlp<0>, grid, block, 0, st, d, L, feat, rows, y, fmode);
-    else
-        hipLaunchKernelGGL(k_mlp<1>, grid, block, 0, st, d, L, feat, rows, y, fmode);
-    return hipGetLastError();
-}
-
-__global__ void k_norm_bf16(Mlp m, const float *__restrict__ x, int rows, uint16_t *__restrict__ out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * kFeat) return;
-    const int c = i % kFeat;
-    out[i] = to_bf16(clipz((x[i] - m.xmu[c]) * m.xrs[c]));
-}
-
-hipError_t launch_norm_bf16(const Learn &L, const float *xraw, int rows, uint16_t *out, hipStream_t st)
-{
-    if (rows <= 0) return hipSuccess;
-    const int total = rows * kFeat;
-    hipLaunchKernelGGL(k_norm_bf16, dim3((total + 255) / 256), dim3(256), 0, st, L.mlp, xraw, rows, out);
+    hipLaunchKernelGGL(k_mlp_rows, dim3((rows + 255) / 256), dim3(256), 0, st, L.mlp, xraw, rows, y);
     return hipGetLastError();
 }
 
