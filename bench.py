@@ -203,11 +203,13 @@ def main():
     main_r = results[mode]
     force_ms = main_r["stages"]["force"]  # the traversal (and, learned, the network and the body update)
     achieved = FLOP_PER_INTERACTION * main_r["interactions"] / (force_ms * 1e-3) / 1e12
+    # HBM bytes per launch of the dominant kernel, from the rocprofv3 counter runs
+    # summarized in profiles/ (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "force_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch", {}).get("k_coarse64" if mode == "learned" else "k_force64")
 
     line = {
         "metric": METRIC,
