@@ -1240,6 +1240,120 @@ int nb_mlp_adam(nb_sim *s, const float *x, const float *t, int32_t n, int32_t it
     return host_data_passes(s, x, t, n, iters, true, lr, nullptr, params);
 }
 
+// ---- diagnostics and checkpoints --------------------------------------------
+
+int nb_diagnostics(nb_sim *s, int32_t pe_samples, uint64_t seed, double *out)
+{
+    if (!s || !out) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const int n = s->d.n;
+    Scratch m(s);
+    double *mp = nullptr;
+    int r = m.get(&mp, (size_t)nb::kMomentBlocks * 11);
+    if (r) return r;
+    hipError_t e = nb::launch_moments(s->d, mp, s->stream);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "moments launch: %s", hipGetErrorString(e));
+    std::vector<double> part;
+    r = fetch(s, part, (const double *)mp, (size_t)nb::kMomentBlocks * 11);
+    if (r) return r;
+    double sum[11] = {};
+    for (int b = 0; b < nb::kMomentBlocks; ++b)
+        for (int k = 0; k < 11; ++k) sum[k] += part[(size_t)b * 11 + k];
+    // potential: targets in batches (the direct sum keeps 64 partial sums per target)
+    const bool exact = pe_samples <= 0 || pe_samples >= n;
+    const int ns = exact ? n : pe_samples;
+    std::vector<int32_t> idx((size_t)ns);
+    if (exact) {
+        for (int i = 0; i < n; ++i) idx[i] = i;
+    } else {
+        std::vector<int32_t> perm((size_t)n);
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        uint64_t x = seed;
+        for (int i = 0; i < ns; ++i) std::swap(perm[i], perm[i + (int)(splitmix64(x) % (uint64_t)(n - i))]);
+        std::copy(perm.begin(), perm.begin() + ns, idx.begin());
+    }
+    std::vector<float4> pm;
+    r = fetch(s, pm, (const float4 *)s->d.pm4, (size_t)n);
+    if (r) return r;
+    const int batch = std::min(ns, 65536);
+    r = ensure_check_buffers(s, batch);
+    if (r) return r;
+    double pe = 0.0;
+    for (int b0 = 0; b0 < ns; b0 += batch) {
+        const int nb_ = std::min(batch, ns - b0);
+        std::vector<double4> res;
+        HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data() + b0, (size_t)nb_ * sizeof(int), hipMemcpyHostToDevice,
+                                  s->stream));
+        e = nb::launch_direct64(s->d, s->chk_idx, nb_, s->chk_part, s->chk_out, s->stream);
+        if (e != hipSuccess) return fail(s, NB_E_HIP, "direct-sum launch: %s", hipGetErrorString(e));
+        r = fetch(s, res, (const double4 *)s->chk_out, (size_t)nb_);
+        if (r) return r;
+        for (int i = 0; i < nb_; ++i) pe += (double)pm[idx[b0 + i]].w * res[i].w;
+    }
+    pe *= -0.5 * (double)n / (double)ns;
+    out[0] = sum[4];
+    out[1] = pe;
+    out[2] = sum[1];
+    out[3] = sum[2];
+    out[4] = sum[3];
+    out[5] = sum[5];
+    out[6] = sum[6];
+    out[7] = sum[7];
+    const double inv = sum[0] > 0.0 ? 1.0 / sum[0] : 0.0;
+    out[8] = sum[8] * inv;
+    out[9] = sum[9] * inv;
+    out[10] = sum[10] * inv;
+    out[11] = sum[0];
+    return NB_OK;
+}
+
+int nb_get_masses(nb_sim *s, float *mass)
+{
+    if (!s || !mass) return NB_E_ARG;
+    HIP_TRY(s, hipSetDevice(s->device));
+    std::vector<float4> p;
+    int r = fetch(s, p, (const float4 *)s->d.pm4, (size_t)s->d.n);
+    if (r) return r;
+    for (size_t i = 0; i < p.size(); ++i) mass[i] = p[i].w;
+    return NB_OK;
+}
+
+int nb_restore_state(nb_sim *s, const double *x, const double *y, const double *z, const float *vx,
+                     const float *vy, const float *vz, const float *ax, const float *ay, const float *az,
+                     const float *mass, int32_t step, int32_t max_depth)
+{
+    if (!s) return NB_E_ARG;
+    if (!x || !y || !z || !vx || !vy || !vz || !ax || !ay || !az || !mass)
+        return fail(s, NB_E_ARG, "null state array");
+    if (step < -1 || max_depth < 1 || max_depth > NB_MAX_DEPTH) return fail(s, NB_E_ARG, "invalid step or depth");
+    HIP_TRY(s, hipSetDevice(s->device));
+    const size_t n = (size_t)s->d.n;
+    std::vector<float> fx(n), fy(n), fz(n);
+    for (size_t i = 0; i < n; ++i) {
+        fx[i] = (float)x[i];
+        fy[i] = (float)y[i];
+        fz[i] = (float)z[i];
+    }
+    int r = nb_set_state(s, fx.data(), fy.data(), fz.data(), vx, vy, vz, mass);
+    if (r) return r;
+    nb::Dev &d = s->d;
+    {
+        std::vector<float4> acc(n);
+        for (size_t i = 0; i < n; ++i) acc[i] = make_float4(ax[i], ay[i], az[i], 0.f);
+        HIP_TRY(s, hipMemcpy(d.acc4, acc.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    if (d.fp64) {
+        std::vector<double4> p64(n);
+        for (size_t i = 0; i < n; ++i) p64[i] = make_double4(x[i], y[i], z[i], 0.0);
+        HIP_TRY(s, hipMemcpy(d.pos64, p64.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(s, hipMemcpyAsync(&d.ctl->step, &step, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(&d.ctl->max_depth, &max_depth, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
 int nb_get_counters(nb_sim *s, nb_counters *out)
 {
     if (!s || !out) return NB_E_ARG;

@@ -220,7 +220,8 @@ hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 direct sum on sampled bodies (the error yardstick; G = 1).
+// FP64 direct sum on sampled bodies (the error yardstick; G = 1), with the
+// softened potential sum_{j != i} m_j / sqrt(r^2 + eps^2) in the w component.
 // Block (t, s) sums the bodies [s * chunk, (s + 1) * chunk) for 64 sampled
 // targets: 4 threads per target stride through the chunk, whose bodies are
 // staged through LDS once for all 64 targets. Partial sums per (target,
@@ -243,15 +244,17 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
     const int sub = threadIdx.x / kDirTargets;      // 0..3: which quarter of each tile
     const int tgt = t0 + tl;
     double tx = 0.0, ty = 0.0, tz = 0.0;
+    int self = -1;
     if (tgt < ns) {
         const int b = idx[tgt];
         if (b >= 0 && b < n) {
             const float4 p = d.pm4[b];
             tx = p.x; ty = p.y; tz = p.z;
+            self = b;
         }
     }
     const double eps2 = (double)d.eps2;
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
     for (int base = lo; base < hi; base += kDirTile) {
         __syncthreads();
         const int j = base + threadIdx.x;
@@ -267,15 +270,16 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
             ax = fma(dx, f, ax);
             ay = fma(dy, f, ay);
             az = fma(dz, f, az);
+            phi = fma((double)s.w, base + k != self ? ri : 0.0, phi);
         }
     }
-    red[threadIdx.x] = make_double4(ax, ay, az, 0.0);
+    red[threadIdx.x] = make_double4(ax, ay, az, phi);
     __syncthreads();
     if (sub == 0 && tgt < ns) {
         double4 r = red[tl];
         for (int k = 1; k < 4; ++k) {
             const double4 o = red[tl + k * kDirTargets];
-            r.x += o.x; r.y += o.y; r.z += o.z;
+            r.x += o.x; r.y += o.y; r.z += o.z; r.w += o.w;
         }
         part[(size_t)tgt * kDirChunks + blockIdx.y] = r;
     }
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void k_direct64_reduce(const double4 *part, in
     double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int k = 0; k < kDirChunks; ++k) {
         const double4 o = part[(size_t)t * kDirChunks + k];
-        r.x += o.x; r.y += o.y; r.z += o.z;
+        r.x += o.x; r.y += o.y; r.z += o.z; r.w += o.w;
     }
     out[t] = r;
 }
@@ -301,6 +305,58 @@ hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, 
     hipLaunchKernelGGL(k_direct64, grid, dim3(256), 0, st, d, idx, ns, part);
     hipLaunchKernelGGL(k_direct64_reduce, dim3((ns + 255) / 256), dim3(256), 0, st, (const double4 *)part, ns,
                        out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 sums over all bodies for the diagnostics: per block (grid-stride over
+// bodies, fixed grid) the mass, momentum, kinetic energy, angular momentum
+// about the origin and first moment of mass; the host adds the blocks in order.
+// ---------------------------------------------------------------------------
+constexpr int kMomTerms = 11; // m, p(3), ke, L(3), m x(3)
+
+__global__ __launch_bounds__(256) void k_moments(Dev d, double *part)
+{
+    __shared__ double red[kMomTerms][256];
+    double acc[kMomTerms];
+#pragma unroll
+    for (int k = 0; k < kMomTerms; ++k) acc[k] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < d.n; i += kMomentBlocks * 256) {
+        const float4 p = d.pm4[i];
+        const float4 v = d.vel4[i];
+        double x = p.x, y = p.y, z = p.z;
+        if (d.fp64) {
+            const double4 q = d.pos64[i];
+            x = q.x; y = q.y; z = q.z;
+        }
+        const double m = p.w, vx = v.x, vy = v.y, vz = v.z;
+        acc[0] += m;
+        acc[1] += m * vx;
+        acc[2] += m * vy;
+        acc[3] += m * vz;
+        acc[4] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        acc[5] += m * (y * vz - z * vy);
+        acc[6] += m * (z * vx - x * vz);
+        acc[7] += m * (x * vy - y * vx);
+        acc[8] += m * x;
+        acc[9] += m * y;
+        acc[10] += m * z;
+    }
+#pragma unroll
+    for (int k = 0; k < kMomTerms; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < kMomTerms; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < kMomTerms) part[(size_t)blockIdx.x * kMomTerms + threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_moments(const Dev &d, double *part, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_moments, dim3(kMomentBlocks), dim3(256), 0, st, d, part);
     return hipGetLastError();
 }
 

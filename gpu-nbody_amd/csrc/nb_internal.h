@@ -111,9 +111,13 @@ hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)
 // mode: 0 = store accelerations (reference semantics), 1 = also integrate
 // (fused kick-drift-kick), 2 = probe: write accelerations to d.probe only
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);
-// FP64 direct-sum accelerations of the bodies idx[0..ns) (G = 1), into out
+// FP64 direct-sum accelerations of the bodies idx[0..ns) (G = 1), into out;
+// out[i].w = sum_{j != idx[i]} m_j / sqrt(r^2 + eps^2) (the softened potential, negated)
 hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, hipStream_t st);
 hipError_t launch_integrate(const Dev &d, hipStream_t st);
+// FP64 per-block sums of m, m v, m v^2 / 2, m x cross v, m x (11 doubles per block)
+constexpr int kMomentBlocks = 512;
+hipError_t launch_moments(const Dev &d, double *part, hipStream_t st);
 hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st);
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st);
 

@@ -154,6 +154,9 @@ def load() -> C.CDLL:
     L.nb_mlp_forward.argtypes = [vp, _f32p, C.c_int32, _f32p]
     L.nb_mlp_loss_grad.argtypes = [vp, _f32p, _f32p, C.c_int32, C.POINTER(C.c_float), _f32p]
     L.nb_mlp_adam.argtypes = [vp, _f32p, _f32p, C.c_int32, C.c_int32, C.c_float, _f32p]
+    L.nb_diagnostics.argtypes = [vp, C.c_int32, C.c_uint64, vp]
+    L.nb_get_masses.argtypes = [vp, vp]
+    L.nb_restore_state.argtypes = [vp] + [vp] * 10 + [C.c_int32, C.c_int32]
     L.nb_last_error.argtypes = [vp]
     L.nb_last_error.restype = C.c_char_p
     L.nb_status_string.argtypes = [C.c_int]
@@ -163,7 +166,8 @@ def load() -> C.CDLL:
                  "nb_get_sorted", "nb_get_paths", "nb_get_cells", "nb_get_counters",
                  "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error",
                  "nb_train", "nb_set_model", "nb_get_model", "nb_coarse_features",
-                 "nb_mlp_forward", "nb_mlp_loss_grad", "nb_mlp_adam"):
+                 "nb_mlp_forward", "nb_mlp_loss_grad", "nb_mlp_adam", "nb_diagnostics",
+                 "nb_get_masses", "nb_restore_state"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -319,3 +319,38 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
         _check(self._lib, self._h, self._lib.nb_mlp_adam(self._h, x.reshape(-1), t.reshape(-1), len(x),
                                                          int(iters), float(lr), p))
         return p
+
+    # -- diagnostics and checkpoints ---------------------------------------------
+    def diagnostics(self, pe_samples: int = 0, seed: int = 1) -> dict:
+        """FP64 kinetic and softened potential energy (exact when pe_samples <= 0),
+        momentum, angular momentum, centre of mass and total mass."""
+        out = np.zeros(12, np.float64)
+        _check(self._lib, self._h, self._lib.nb_diagnostics(self._h, int(pe_samples), int(seed), ptr(out)))
+        return dict(kinetic=out[0], potential=out[1], energy=out[0] + out[1], momentum=out[2:5].copy(),
+                    angular_momentum=out[5:8].copy(), centre_of_mass=out[8:11].copy(), mass=out[11])
+
+    def masses(self) -> np.ndarray:
+        m = np.zeros(self.nbodies, np.float32)
+        _check(self._lib, self._h, self._lib.nb_get_masses(self._h, ptr(m)))
+        return m
+
+    def checkpoint(self, path: str) -> None:
+        """Write the full dynamic state (npz): positions (FP64 when kept), velocities,
+        last accelerations, masses, step counter and running maximum depth."""
+        c = self.counters()
+        x, y, z, vx, vy, vz = self.state()
+        pos = (self.positions64() if self._params.flags & _lib.NB_FP64_POSITIONS
+               else np.stack([x, y, z], 1).astype(np.float64))
+        np.savez(path, pos=pos, vel=np.stack([vx, vy, vz], 1), acc=self.accel(), m=self.masses(),
+                 step=np.int32(c.step), max_depth=np.int32(c.max_depth))
+
+    def restore(self, path: str) -> None:
+        """Resume from a checkpoint written by checkpoint()."""
+        f = np.load(path)
+        pos, vel, acc = (np.ascontiguousarray(f[k]) for k in ("pos", "vel", "acc"))
+        cols = [np.ascontiguousarray(pos[:, k], dtype=np.float64) for k in range(3)]
+        cols += [np.ascontiguousarray(vel[:, k], dtype=np.float32) for k in range(3)]
+        cols += [np.ascontiguousarray(acc[:, k], dtype=np.float32) for k in range(3)]
+        cols.append(np.ascontiguousarray(f["m"], dtype=np.float32))
+        _check(self._lib, self._h, self._lib.nb_restore_state(self._h, *[ptr(a) for a in cols],
+                                                              int(f["step"]), int(f["max_depth"])))

@@ -292,3 +292,123 @@ class FunctionUniverse:
         st = self.fn(nbodies, **self.kw)
         for dst, src in zip((bodiesX, bodiesY, bodiesZ, velX, velY, velZ, bodiesMass), st):
             dst[offset:offset + nbodies] = src
+
+
+# ---- .universe files (UniverseSerializer.java:10-39, SerializedUniverseGenerator.java:21-53) ----
+#
+# The reference writes a universe with java.io.ObjectOutputStream: writeInt(N)
+# (a 4-byte block-data record), then the seven float[] arrays x, y, z, vx, vy,
+# vz, m as serialized objects (big-endian floats). An older layout found in the
+# reference's repository root stores N as a boxed java.lang.Integer object and
+# arrays longer than N (the first N entries are the bodies). Both are read here
+# with a small reader of exactly these stream elements.
+
+_STREAM_MAGIC = b"\xac\xed\x00\x05"
+_TC_NULL, _TC_REFERENCE, _TC_CLASSDESC, _TC_OBJECT = 0x70, 0x71, 0x72, 0x73
+_TC_ARRAY, _TC_BLOCKDATA, _TC_ENDBLOCKDATA = 0x75, 0x77, 0x78
+
+
+class _JavaStream:
+    def __init__(self, data: bytes):
+        if data[:4] != _STREAM_MAGIC:
+            raise ValueError("not a Java serialization stream")
+        self.b, self.p = data, 4
+
+    def u1(self) -> int:
+        v = self.b[self.p]
+        self.p += 1
+        return v
+
+    def take(self, n: int) -> bytes:
+        if self.p + n > len(self.b):
+            raise ValueError("truncated universe file")
+        v = self.b[self.p:self.p + n]
+        self.p += n
+        return v
+
+    def i4(self) -> int:
+        return int.from_bytes(self.take(4), "big", signed=True)
+
+    def classdesc(self) -> str:
+        tc = self.u1()
+        if tc == _TC_REFERENCE:
+            self.take(4)
+            return ""
+        if tc == _TC_NULL:
+            return ""
+        if tc != _TC_CLASSDESC:
+            raise ValueError(f"unexpected type code 0x{tc:02x} for a class descriptor")
+        name = self.take(int.from_bytes(self.take(2), "big")).decode("utf-8")
+        self.take(8)                      # serialVersionUID
+        self.u1()                         # flags
+        for _ in range(int.from_bytes(self.take(2), "big")):
+            code = chr(self.u1())
+            self.take(int.from_bytes(self.take(2), "big"))      # field name
+            if code in "L[":              # object field: its type as a string
+                if self.u1() == 0x74:     # TC_STRING
+                    self.take(int.from_bytes(self.take(2), "big"))
+                else:
+                    self.take(4)          # TC_REFERENCE handle
+        if self.u1() != _TC_ENDBLOCKDATA:
+            raise ValueError("class annotations are not supported")
+        self.classdesc()                  # superclass
+        return name
+
+    def count(self) -> int:
+        tc = self.u1()
+        if tc == _TC_BLOCKDATA:           # writeInt
+            if self.u1() != 4:
+                raise ValueError("unexpected block-data length")
+            return self.i4()
+        if tc == _TC_OBJECT:              # boxed Integer (older layout)
+            self.classdesc()
+            return self.i4()
+        raise ValueError(f"unexpected type code 0x{tc:02x} for the body count")
+
+    def float_array(self) -> np.ndarray:
+        if self.u1() != _TC_ARRAY:
+            raise ValueError("expected a float[] array")
+        self.classdesc()
+        n = self.i4()
+        return np.frombuffer(self.take(4 * n), dtype=">f4").astype(np.float32)
+
+
+def read_universe(path: str) -> State:
+    """Bodies of a .universe file as float32 (x, y, z, vx, vy, vz, m)."""
+    with open(path, "rb") as f:
+        js = _JavaStream(f.read())
+    n = js.count()
+    arrays = [js.float_array() for _ in range(7)]
+    if n < 0 or any(len(a) < n for a in arrays):
+        raise ValueError("universe arrays are shorter than the body count")
+    return tuple(np.ascontiguousarray(a[:n]) for a in arrays)
+
+
+def write_universe(path: str, state: State) -> None:
+    """Write bodies in the layout of UniverseSerializer.java:10-39 (writeInt(N),
+    then the seven float[] arrays), readable by read_universe and by the
+    reference's SerializedUniverseGenerator."""
+    arrays = [np.asarray(a, np.float32) for a in state]
+    n = len(arrays[0])
+    out = bytearray(_STREAM_MAGIC)
+    out += bytes([_TC_BLOCKDATA, 4]) + n.to_bytes(4, "big", signed=True)
+    for k, a in enumerate(arrays):
+        out.append(_TC_ARRAY)
+        if k == 0:
+            out += bytes([_TC_CLASSDESC]) + (2).to_bytes(2, "big") + b"[F"
+            out += bytes.fromhex("0b9c818922e00c42")        # serialVersionUID of float[]
+            out += bytes([0x02]) + (0).to_bytes(2, "big")   # SC_SERIALIZABLE, no fields
+            out += bytes([_TC_ENDBLOCKDATA, _TC_NULL])
+        else:
+            out += bytes([_TC_REFERENCE]) + (0x7E0000).to_bytes(4, "big")   # handle of the float[] descriptor
+        out += len(a).to_bytes(4, "big", signed=True) + a.astype(">f4").tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+class FileUniverse(ArrayUniverse):
+    """SerializedUniverseGenerator (SerializedUniverseGenerator.java:21-53): the
+    bodies of a .universe file."""
+
+    def __init__(self, path: str):
+        super().__init__(read_universe(path))

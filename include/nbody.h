@@ -254,6 +254,31 @@ int nb_mlp_loss_grad(nb_sim *sim, const float *x, const float *t, int32_t n, flo
 int nb_mlp_adam(nb_sim *sim, const float *x, const float *t, int32_t n, int32_t iters, float lr,
                 float *params);
 
+/* ---- diagnostics and checkpoints (SURVEY-listed follow-ups; the reference's
+ *      printEnergy, GPUBarnesHutNBodySimulation.java:305-365, counts every pair
+ *      twice and ignores the softening) -------------------------------------- */
+
+/* FP64 sums at the current state (positions in FP64 when NB_FP64_POSITIONS):
+ * out12[0] kinetic energy, out12[1] potential energy
+ * -1/2 sum_i sum_{j != i} m_i m_j / sqrt(|x_i - x_j|^2 + eps^2) over the FP32
+ * positions the force sees -- exact (all N bodies, O(N^2) on the device) when
+ * pe_samples <= 0 or >= N, otherwise estimated from pe_samples distinct bodies
+ * drawn with `seed` and scaled by N / pe_samples; out12[2..4] momentum,
+ * out12[5..7] angular momentum about the origin, out12[8..10] centre of mass,
+ * out12[11] total mass. */
+int nb_diagnostics(nb_sim *sim, int32_t pe_samples, uint64_t seed, double *out12);
+
+/* Masses by original body index (the missing read-back for a checkpoint). */
+int nb_get_masses(nb_sim *sim, float *m);
+
+/* Resume from a checkpoint: positions (FP64; rounded to FP32 for the tree),
+ * velocities, last accelerations and masses by original body index, and the
+ * step counter and running maximum depth (nb_get_counters). The next nb_step
+ * continues exactly as the interrupted run would have. */
+int nb_restore_state(nb_sim *sim, const double *x, const double *y, const double *z, const float *vx,
+                     const float *vy, const float *vz, const float *ax, const float *ay, const float *az,
+                     const float *m, int32_t step, int32_t max_depth);
+
 /* Per-stage HIP-event timing of the following nb_step calls (0 = off). */
 int nb_set_timing(nb_sim *sim, int32_t enabled);
 
