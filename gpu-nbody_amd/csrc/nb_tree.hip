@@ -443,122 +443,115 @@ __global__ __launch_bounds__(256) void k_level_scatter(Dev d)
 
 // ---------------------------------------------------------------------------
 // Stage 3: summarize (summarizetree.cl:15-179), one launch per level from the
-// deepest up. A cell's centre of mass is summed in FP64 in child-slot order
-// from its children's results of the previous launch; children are compacted
-// to the front (summarizetree.cl:75-81) into the traversal list.
+// deepest up. Eight lanes per cell, one per child slot, so the children's
+// records are fetched in parallel. A cell's centre of mass is summed in FP64
+// from its children's results of the previous launch, over the eight slots
+// (an empty slot adds 0) in the fixed pairwise order
+// ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)), which the xor-shuffle
+// reduction produces identically on every lane. Children are compacted to
+// the front (summarizetree.cl:75-81) into the traversal list.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double sum8(double v)
+{
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
 template <bool QUAD>
 __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int cells)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cells) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 3, k = t & 7;
+    const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
     const int n = d.n;
     const uint32_t ep = d.ctl->epoch;
-    const int c = d.lvl_list[d.ctl->lvl_start[level] + i];
-    const unsigned long long *sl = d.slot + (size_t)c * 8;
-    int ch[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ch[k] = dec_slot(sl[k], ep);
-    double m = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-    int cnt = 0, used = 0;
-    int *out = d.cchild + (size_t)c * 8;
-    // learned mode: each child's mass and centre stay in registers for the
-    // second-moment pass, and the children's own moments are summed here
-    double cm[QUAD ? 8 : 1], cpx[QUAD ? 8 : 1], cpy[QUAD ? 8 : 1], cpz[QUAD ? 8 : 1];
-    double qs[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double eps2 = (double)d.eps2;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int v = ch[k];
-        if (QUAD) cm[k] = cpx[k] = cpy[k] = cpz[k] = 0.0;
-        if (v < 0) continue;
-        float4 *cd = d.cdata + (size_t)c * 8 + used;
-        float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = make_float4(0.f, 0.f, 0.f, 0.f);
-        double mk, xk, yk, zk;
-        if (v < n) {
-            const float4 b = d.pm4[v];
-            *cd = b;
-            mk = b.w;
-            xk = b.x;
-            yk = b.y;
-            zk = b.z;
-            cnt += 1;
-        } else {
-            const double4 q = d.com64[v - n];
-            *cd = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-            mk = q.w;
-            xk = q.x;
-            yk = q.y;
-            zk = q.z;
-            cnt += d.count[v - n];
-            if (QUAD) {
-                const double4 *cq = reinterpret_cast<const double4 *>(d.cq64 + (size_t)(v - n) * 8);
-                const double4 q0 = cq[0], q1 = cq[1];
-                qs[0] += q0.x;
-                qs[1] += q0.y;
-                qs[2] += q0.z;
-                qs[3] += q0.w;
-                qs[4] += q1.x;
-                qs[5] += q1.y;
-                qs[6] += q1.z;
-                qa = make_float4((float)q0.x, (float)q0.y, (float)q0.z, (float)(q1.z * eps2));
-                qb = make_float4((float)q0.w, (float)q1.x, (float)q1.y, 0.f);
-            }
-        }
-        if (QUAD) {
-            d.cqa[(size_t)c * 8 + used] = qa;
-            d.cqb[(size_t)c * 8 + used] = qb;
-        }
-        if (QUAD) {
-            cm[k] = mk;
-            cpx[k] = xk;
-            cpy[k] = yk;
-            cpz[k] = zk;
-        }
-        out[used++] = v;
-        m += mk;
-        sx = fma(mk, xk, sx);
-        sy = fma(mk, yk, sy);
-        sz = fma(mk, zk, sz);
+    int c = 0, v = -1;
+    if (live) {
+        c = d.lvl_list[d.ctl->lvl_start[level] + i];
+        v = dec_slot(d.slot[(size_t)c * 8 + k], ep);
     }
-    for (int k = used; k < 8; ++k) {
-        out[k] = -1;
-        d.cdata[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    double m = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    int cnt = 0;
+    float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+    double q[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; // the child's own second moments
+    if (v >= 0 && v < n) {
+        rec = d.pm4[v];
+        m = rec.w;
+        x = rec.x;
+        y = rec.y;
+        z = rec.z;
+        cnt = 1;
+    } else if (v >= n) {
+        const double4 cc = d.com64[v - n];
+        rec = make_float4((float)cc.x, (float)cc.y, (float)cc.z, (float)cc.w);
+        m = cc.w;
+        x = cc.x;
+        y = cc.y;
+        z = cc.z;
+        cnt = d.count[v - n];
         if (QUAD) {
-            d.cqa[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            d.cqb[(size_t)c * 8 + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const double4 *cq = reinterpret_cast<const double4 *>(d.cq64 + (size_t)(v - n) * 8);
+            const double4 q0 = cq[0], q1 = cq[1];
+            q[0] = q0.x; q[1] = q0.y; q[2] = q0.z; q[3] = q0.w;
+            q[4] = q1.x; q[5] = q1.y; q[6] = q1.z;
         }
     }
+    // compaction: position of this child among the cell's non-empty slots;
+    // empty slots fill the tail
+    const int gl = (int)(threadIdx.x & 63) & ~7;
+    const unsigned gm = (unsigned)((__ballot(v >= 0) >> gl) & 0xffull);
+    const int used = __popc(gm), below = __popc(gm & ((1u << k) - 1u));
+    const int at = v >= 0 ? below : used + (k - below);
+    const double sm = sum8(m), sx = sum8(m * x), sy = sum8(m * y), sz = sum8(m * z);
+    int scnt = cnt;
+    scnt += __shfl_xor(scnt, 1);
+    scnt += __shfl_xor(scnt, 2);
+    scnt += __shfl_xor(scnt, 4);
     // a cell of zero mass gets its centre at the origin instead of the
     // reference's 0 * inf = NaN (summarizetree.cl:161-167)
-    const double inv = m > 0.0 ? 1.0 / m : 0.0;
-    const double4 com = make_double4(sx * inv, sy * inv, sz * inv, m);
-    d.com64[c] = com;
-    d.count[c] = cnt;
+    const double inv = sm > 0.0 ? 1.0 / sm : 0.0;
+    const double4 com = make_double4(sx * inv, sy * inv, sz * inv, sm);
+    if (live) {
+        d.cchild[(size_t)c * 8 + at] = v;
+        d.cdata[(size_t)c * 8 + at] = rec;
+        if (k == 0) {
+            d.com64[c] = com;
+            d.count[c] = scnt;
+        }
+    }
     if (QUAD) {
         // Second moments about the centre of mass for the softened quadrupole
         // of the learned mode: the traceless tensor Q = sum m (3 s s^T - |s|^2 I)
         // and the trace T = sum m |s|^2 (s = offset from the centre of mass):
-        // the children's own moments (summed above) plus, by the parallel-axis
-        // rule, each child's mass at its offset from this cell's centre of mass.
-        // Each child's moments also sit next to its (x, y, z, m) for the
-        // traversal, as (Qxx, Qyy, Qzz, T eps^2) and (Qxy, Qxz, Qyz, 0).
+        // per slot, the child's own moments plus, by the parallel-axis rule,
+        // its mass at its offset from this cell's centre of mass, then summed
+        // over the slots in the same pairwise order. Each child's moments also
+        // sit next to its (x, y, z, m) for the traversal, as
+        // (Qxx, Qyy, Qzz, T eps^2) and (Qxy, Qxz, Qyz, 0).
+        const double dx = x - com.x, dy = y - com.y, dz = z - com.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double t7[7];
+        t7[0] = q[0] + m * (3.0 * dx * dx - r2);
+        t7[1] = q[1] + m * (3.0 * dy * dy - r2);
+        t7[2] = q[2] + m * (3.0 * dz * dz - r2);
+        t7[3] = q[3] + m * 3.0 * dx * dy;
+        t7[4] = q[4] + m * 3.0 * dx * dz;
+        t7[5] = q[5] + m * 3.0 * dy * dz;
+        t7[6] = q[6] + m * r2;
 #pragma unroll
-        for (int k = 0; k < (QUAD ? 8 : 0); ++k) {
-            const double mx = cm[k];
-            const double dx = cpx[k] - com.x, dy = cpy[k] - com.y, dz = cpz[k] - com.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            qs[0] += mx * (3.0 * dx * dx - r2);
-            qs[1] += mx * (3.0 * dy * dy - r2);
-            qs[2] += mx * (3.0 * dz * dz - r2);
-            qs[3] += mx * 3.0 * dx * dy;
-            qs[4] += mx * 3.0 * dx * dz;
-            qs[5] += mx * 3.0 * dy * dz;
-            qs[6] += mx * r2;
+        for (int j = 0; j < 7; ++j) t7[j] = sum8(t7[j]);
+        if (live) {
+            const double eps2 = (double)d.eps2;
+            d.cqa[(size_t)c * 8 + at] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[6] * eps2));
+            d.cqb[(size_t)c * 8 + at] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
+            if (k == 0) {
+                double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);
+                cq[0] = make_double4(t7[0], t7[1], t7[2], t7[3]);
+                cq[1] = make_double4(t7[4], t7[5], t7[6], 0.0);
+            }
         }
-        double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);
-        cq[0] = make_double4(qs[0], qs[1], qs[2], qs[3]);
-        cq[1] = make_double4(qs[4], qs[5], qs[6], 0.0);
     }
 }
 
@@ -571,27 +564,38 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cells) return;
+    // eight lanes per cell, one per compacted child; each lane's offset is the
+    // exclusive prefix sum of its siblings' sizes (a body counts 1)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 3, k = t & 7;
+    const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
     const int n = d.n;
     Ctl *ctl = d.ctl;
-    const int c = d.lvl_list[ctl->lvl_start[level] + i];
-    int st = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
-    if (level == 0) d.start[c] = 0;
-    const int *cc = d.cchild + (size_t)c * 8;
+    int c = 0, v = -1, base = 0;
+    if (live) {
+        c = d.lvl_list[ctl->lvl_start[level] + i];
+        base = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
+        v = d.cchild[(size_t)c * 8 + k];
+    }
+    const int size = v >= n ? d.count[v - n] : (v >= 0 ? 1 : 0);
+    int incl = size; // inclusive prefix over the 8 lanes of this cell
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int v = cc[k];
-        if (v < 0) break;
-        if (v >= n) { // sort.cl:223-237
-            d.start[v - n] = st;
-            st += d.count[v - n];
-        } else { // sort.cl:238-245
-            if (st < 0 || st >= n) { set_error(ctl, kErrSync); return; }
-            d.sorted[st] = v;
-            d.bnode[st] = d.cdata[(size_t)c * 8 + k];
-            ++st;
+    for (int o = 1; o < 8; o <<= 1) {
+        const int u = __shfl_up(incl, o, 8);
+        if (k >= o) incl += u;
+    }
+    const int st = base + incl - size;
+    if (!live) return;
+    if (level == 0 && k == 0) d.start[c] = 0;
+    if (v >= n) { // sort.cl:223-237
+        d.start[v - n] = st;
+    } else if (v >= 0) { // sort.cl:238-245
+        if (st < 0 || st >= n) {
+            set_error(ctl, kErrSync);
+            return;
         }
+        d.sorted[st] = v;
+        d.bnode[st] = d.cdata[(size_t)c * 8 + k];
     }
 }
 
@@ -697,16 +701,17 @@ hipError_t launch_levels(const Dev &d, hipStream_t st)
 hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st)
 {
     if (cells <= 0) return hipSuccess;
+    const dim3 grid((cells * 8 + 255) / 256); // eight lanes per cell
     if (d.quad)
-        hipLaunchKernelGGL(k_summarize_level<true>, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+        hipLaunchKernelGGL(k_summarize_level<true>, grid, dim3(256), 0, st, d, level, cells);
     else
-        hipLaunchKernelGGL(k_summarize_level<false>, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+        hipLaunchKernelGGL(k_summarize_level<false>, grid, dim3(256), 0, st, d, level, cells);
     return hipGetLastError();
 }
 hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)
 {
     if (cells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_level, dim3((cells + 255) / 256), dim3(256), 0, st, d, level, cells);
+    hipLaunchKernelGGL(k_sort_level, dim3((cells * 8 + 255) / 256), dim3(256), 0, st, d, level, cells);
     return hipGetLastError();
 }
 hipError_t launch_integrate(const Dev &d, hipStream_t st)

@@ -295,19 +295,33 @@ int orc_build(orc_sim *s)
 
 /* ---- stage 3: summarize, summarizetree.cl:55-178, cells in ascending index
  * order (children always have lower indices than their parent, since cells
- * are allocated downward), children compacted to the front (:75-81). ----- */
+ * are allocated downward), children compacted to the front (:75-81).
+ *
+ * FP64 mode (the GPU design): the sums run over the eight octant slots (an
+ * empty slot adds 0) in the fixed pairwise order ((s0+s1)+(s2+s3))+((s4+s5)+
+ * (s6+s7)); a body contributes (m, m*x, ...) from its FP32 values, a cell its
+ * FP64 centre of mass. The same pass computes every cell's second moments
+ * about its centre of mass for the learned mode (see orc_moments). ------- */
+static double pair8(const double *a)
+{
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 void orc_summarize(orc_sim *s)
 {
     const int n = s->n, nn = s->nn;
     float *X = s->x, *Y = s->y, *Z = s->z, *M = s->m;
     int *child = s->child;
     const int fp32 = (s->flags & ORC_COM_FP32) != 0;
+    if (!fp32 && !s->cq) s->cq = (double *)calloc(8 * ((size_t)nn + 1), sizeof(double));
     for (int node = s->bottom; node <= nn; ++node) {
         int used = 0, cnt = 0;
-        double m64 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        double mk[8], xk[8], yk[8], zk[8], qk[8][7];
         float m32 = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (int k = 0; k < 8; ++k) {
             const int c = child[8 * node + k];
+            mk[k] = xk[k] = yk[k] = zk[k] = 0.0;
+            for (int j = 0; j < 7; ++j) qk[k][j] = 0.0;
             if (c < 0) continue;
             if (k != used) { /* move to front, :77-81 */
                 child[8 * node + k] = -1;
@@ -322,8 +336,8 @@ void orc_summarize(orc_sim *s)
                     fx = fmaf(X[c], mc, fx); fy = fmaf(Y[c], mc, fy); fz = fmaf(Z[c], mc, fz);
                 } else {
                     const double *cc = s->com + 4 * (size_t)c;
-                    m64 += cc[3];
-                    sx = fma(cc[3], cc[0], sx); sy = fma(cc[3], cc[1], sy); sz = fma(cc[3], cc[2], sz);
+                    mk[k] = cc[3]; xk[k] = cc[0]; yk[k] = cc[1]; zk[k] = cc[2];
+                    for (int j = 0; j < 7; ++j) qk[k][j] = s->cq[8 * (size_t)c + j];
                 }
             } else {
                 cnt += 1;
@@ -332,10 +346,7 @@ void orc_summarize(orc_sim *s)
                     m32 += mc;
                     fx = fmaf(X[c], mc, fx); fy = fmaf(Y[c], mc, fy); fz = fmaf(Z[c], mc, fz);
                 } else {
-                    m64 += (double)mc;
-                    sx = fma((double)mc, (double)X[c], sx);
-                    sy = fma((double)mc, (double)Y[c], sy);
-                    sz = fma((double)mc, (double)Z[c], sz);
+                    mk[k] = (double)mc; xk[k] = (double)X[c]; yk[k] = (double)Y[c]; zk[k] = (double)Z[c];
                 }
             }
         }
@@ -349,10 +360,29 @@ void orc_summarize(orc_sim *s)
             M[node] = m32;
             cc[0] = X[node]; cc[1] = Y[node]; cc[2] = Z[node]; cc[3] = m32;
         } else {
+            double px[8], py[8], pz[8];
+            for (int k = 0; k < 8; ++k) { px[k] = mk[k] * xk[k]; py[k] = mk[k] * yk[k]; pz[k] = mk[k] * zk[k]; }
+            const double m64 = pair8(mk), sx = pair8(px), sy = pair8(py), sz = pair8(pz);
             const double inv = m64 > 0.0 ? 1.0 / m64 : 0.0;
             cc[0] = sx * inv; cc[1] = sy * inv; cc[2] = sz * inv; cc[3] = m64;
             X[node] = (float)cc[0]; Y[node] = (float)cc[1]; Z[node] = (float)cc[2];
             M[node] = (float)m64; /* the ready flag of :172, as a value */
+            /* second moments: per slot, the child's own plus its mass at its
+             * offset from this centre of mass (parallel-axis rule) */
+            double t[7][8];
+            for (int k = 0; k < 8; ++k) {
+                const double dx = xk[k] - cc[0], dy = yk[k] - cc[1], dz = zk[k] - cc[2];
+                const double r2 = dx * dx + dy * dy + dz * dz, mx = mk[k];
+                t[0][k] = qk[k][0] + mx * (3.0 * dx * dx - r2);
+                t[1][k] = qk[k][1] + mx * (3.0 * dy * dy - r2);
+                t[2][k] = qk[k][2] + mx * (3.0 * dz * dz - r2);
+                t[3][k] = qk[k][3] + mx * 3.0 * dx * dy;
+                t[4][k] = qk[k][4] + mx * 3.0 * dx * dz;
+                t[5][k] = qk[k][5] + mx * 3.0 * dy * dz;
+                t[6][k] = qk[k][6] + mx * r2;
+            }
+            for (int j = 0; j < 7; ++j) s->cq[8 * (size_t)node + j] = pair8(t[j]);
+            s->cq[8 * (size_t)node + 7] = 0.0;
         }
     }
 }
@@ -818,11 +848,11 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
 
 /* ---- coarse traversal of the learned correction (no reference counterpart)
  *
- * orc_moments: after summarize, every cell's second moments about its FP64
- * centre of mass, bottom-up in ascending cell order (children first), over the
- * compacted children in slot order: traceless Q = sum m (3 s s^T - |s|^2 I)
- * (xx, yy, zz, xy, xz, yz) and trace T = sum m |s|^2, moved to the parent by
- * the parallel-axis rule. Bodies are point masses at their FP32 positions.
+ * Second moments (computed by orc_summarize in FP64 mode): every cell's
+ * traceless Q = sum m (3 s s^T - |s|^2 I) (xx, yy, zz, xy, xz, yz) and trace
+ * T = sum m |s|^2 about its FP64 centre of mass, moved to the parent by the
+ * parallel-axis rule, summed over the octant slots in the pairwise order of
+ * the centre of mass. Bodies are point masses at their FP32 positions.
  *
  * orc_coarse: the wave-uniform traversal of the force stage, restated per vote
  * group with the opening table of `theta`: accepted cells contribute
@@ -836,40 +866,8 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
  * i * vote + l. */
 void orc_moments(orc_sim *s)
 {
-    const int n = s->n, nn = s->nn;
-    if (!s->cq) s->cq = (double *)calloc(8 * ((size_t)nn + 1), sizeof(double));
-    for (int node = s->bottom; node <= nn; ++node) {
-        const double *cc = s->com + 4 * (size_t)node;
-        double q[7] = {0, 0, 0, 0, 0, 0, 0};
-        /* the children's own moments, then each child's mass at its offset */
-        for (int k = 0; k < 8; ++k) {
-            const int c = s->child[8 * node + k];
-            if (c >= n)
-                for (int j = 0; j < 7; ++j) q[j] += s->cq[8 * (size_t)c + j];
-        }
-        for (int k = 0; k < 8; ++k) {
-            const int c = s->child[8 * node + k];
-            if (c < 0) continue;
-            double mx, px, py, pz;
-            if (c >= n) {
-                const double *c2 = s->com + 4 * (size_t)c;
-                mx = c2[3]; px = c2[0]; py = c2[1]; pz = c2[2];
-            } else {
-                mx = s->m[c]; px = s->x[c]; py = s->y[c]; pz = s->z[c];
-            }
-            const double dx = px - cc[0], dy = py - cc[1], dz = pz - cc[2];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            q[0] += mx * (3.0 * dx * dx - r2);
-            q[1] += mx * (3.0 * dy * dy - r2);
-            q[2] += mx * (3.0 * dz * dz - r2);
-            q[3] += mx * 3.0 * dx * dy;
-            q[4] += mx * 3.0 * dx * dz;
-            q[5] += mx * 3.0 * dy * dz;
-            q[6] += mx * r2;
-        }
-        for (int j = 0; j < 7; ++j) s->cq[8 * (size_t)node + j] = q[j];
-        s->cq[8 * (size_t)node + 7] = 0.0;
-    }
+    /* computed by orc_summarize (FP64 mode); kept for callers that ask */
+    (void)s;
 }
 
 /* second moments of a cell (for tests): out7 = Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, T */
@@ -1046,8 +1044,7 @@ long long orc_coarse(orc_sim *s, float theta, const int *groups, int ngroups, fl
                      double *e2)
 {
     float dq[ORC_MAXDEPTH];
-    if (!s->cq) orc_moments(s);
-    if (s->vote > 256 || s->max_depth > ORC_MAXDEPTH) return -1;
+    if (!s->cq || s->vote > 256 || s->max_depth > ORC_MAXDEPTH) return -1;
     orc_dq_table(s->radius, theta * theta, s->eps2, s->max_depth, dq);
     long long total = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
