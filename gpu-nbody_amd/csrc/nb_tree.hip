@@ -684,10 +684,12 @@ hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st)
 }
 hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
 {
-    // seed the top levels with one block, then every other body
-    const int nseed = d.n >= 65536 ? 4096 : 0;
+    // seed the top levels with 1024 evenly spaced bodies (one per thread), then
+    // every other body. Measured at 4M clustered bodies: no seed 4.0 ms, 4096
+    // seeds on one block 1.9 ms, 1024 seeds on four blocks 1.2 ms.
+    const int nseed = d.n >= 65536 ? 1024 : 0;
     const int stride = nseed ? d.n / nseed : 1;
-    if (nseed) hipLaunchKernelGGL(k_build, dim3(1), dim3(256), 0, st, d, stride, nseed, 1);
+    if (nseed) hipLaunchKernelGGL(k_build, dim3(nseed / 256), dim3(256), 0, st, d, stride, nseed, 1);
     hipLaunchKernelGGL(k_build, dim3(blocks), dim3(256), 0, st, d, stride, nseed, 0);
     return hipGetLastError();
 }
