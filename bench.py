@@ -12,13 +12,15 @@ timed region starts. The metric compares two force methods, each of which
 must reach a median relative force error <= 1e-3 against the FP64 direct sum
 on 4096 sampled bodies:
   plain   Barnes-Hut at theta = 0.4 (the reference's opening test)
-  learned coarse traversal (monopole + softened quadrupole) plus the learned
-          correction, trained on the GPU on 50 steps of theta = 0.4 residuals
-          before the timed region; the coarse opening angle is the largest of
-          0.9, 0.85, 0.825, 0.8, 0.75 whose calibration error (a separate sample)
-          is <= 0.9e-3. The sampled FP64 check (1024 bodies every 10 steps,
-          tolerance 1e-3, plain theta as the fallback) runs inside the timed
-          steps.
+  learned coarse traversal (monopole + softened quadrupole; a cell is taken
+          when |d| >= b / theta_c for its extent b and its left-out term
+          m b^3 / s^5 is within alpha = 0.02 of the body's previous |a|) plus
+          the learned correction, trained on the GPU on 50 steps of
+          theta = 0.4 residuals before the timed region; the coarse angle
+          theta_c is the largest of 0.99, 0.97, 0.95, 0.925, 0.9, 0.85 whose
+          calibration error (a separate sample) is <= 0.9e-3. The sampled FP64
+          check (1024 bodies every 10 steps, tolerance 1e-3, plain theta as the
+          fallback) runs inside the timed steps.
 `value` is the steps/s of the requested mode (learned by default); the other
 mode's steps/s and both errors are in `config`.
 
@@ -38,7 +40,8 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-nbody_amd"))
 METRIC = "steps/s @4M bodies: learned coarse-θ vs plain θ=0.4 at equal 1e-3 force error"
 N_BODIES = 4_194_304
 THETA_PLAIN = 0.4
-THETA_COARSE_CANDIDATES = (0.9, 0.85, 0.825, 0.8, 0.75)
+THETA_COARSE_CANDIDATES = (0.99, 0.97, 0.95, 0.925, 0.9, 0.85)
+ALPHA_COARSE = 0.02
 CALIBRATION_TARGET = 0.9e-3
 ERROR_BAR = 1e-3
 TRAIN = dict(steps=50, groups_per_step=512, iters=4000, batch=4096, seed=1)
@@ -55,14 +58,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(state, mode, theta_coarse=0.9):
+def make_sim(state, mode, theta_coarse=0.95):
     from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
     from nbody_amd.universe import ArrayUniverse
 
     kw = dict(theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote_width=VOTE,
               flags=_lib.NB_FP64_POSITIONS | _lib.NB_FUSED_UPDATE)
     if mode == "learned":
-        kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=theta_coarse, **CHECK)
+        kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=theta_coarse, coarse_alpha=ALPHA_COARSE, **CHECK)
     sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
     sim.init(None)
     return sim
@@ -260,6 +263,7 @@ def main():
         tr = learned_info["train"]
         line["config"].update({
             "theta_coarse": learned_info["theta_coarse"],
+            "alpha_coarse": ALPHA_COARSE,
             "plain_steps_per_s": round(p["steps_per_s"], 4),
             "plain_ms_per_step": round(p["ms_per_step"], 3),
             "speedup_vs_plain": round(main_r["steps_per_s"] / p["steps_per_s"], 3),

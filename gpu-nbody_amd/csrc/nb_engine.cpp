@@ -102,8 +102,13 @@ int validate(const nb_params *p)
     if (!(p->pool_factor >= 1.0f) || p->pool_factor > 64.0f) return NB_E_ARG;
     if (p->mode != NB_MODE_PLAIN && p->mode != NB_MODE_LEARNED && p->mode != NB_MODE_COARSE) return NB_E_ARG;
     if (!(p->coarse_theta > 0.0f) || !std::isfinite(p->coarse_theta)) return NB_E_ARG;
-    // the coarse traversal is wave-uniform: one vote group is one wave64
-    if (p->mode != NB_MODE_PLAIN && (p->vote_width != 64 || !(p->coarse_theta >= p->theta))) return NB_E_ARG;
+    if (!(p->coarse_alpha >= 0.0f) || !std::isfinite(p->coarse_alpha)) return NB_E_ARG;
+    // the coarse traversal is wave-uniform: one vote group is one wave64; its
+    // extent test |d| >= b / theta_c keeps a body out of every cell it accepts
+    // only for theta_c < 1
+    if (p->mode != NB_MODE_PLAIN &&
+        (p->vote_width != 64 || !(p->coarse_theta >= p->theta) || !(p->coarse_theta < 1.0f)))
+        return NB_E_ARG;
     if (!(p->check_tol >= 0.0f) || p->check_every < 0 || p->check_samples < 0) return NB_E_ARG;
     // the traversal stack packs (cell << 6 | depth) into 32 bits
     if ((double)pool_cells(p) > (double)(1 << 26)) return NB_E_ARG;
@@ -167,11 +172,7 @@ int prepare_levels(nb_sim *s)
     return NB_OK;
 }
 
-// Device address of the coarse opening table inside Ctl.
-const float *dq_coarse_ptr(const nb_sim *s)
-{
-    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(s->d.ctl) + offsetof(nb::Ctl, dq_coarse));
-}
+// Device address of the force stage's opening table inside Ctl.
 const float *dq_plain_ptr(const nb_sim *s)
 {
     return reinterpret_cast<const float *>(reinterpret_cast<const char *>(s->d.ctl) + offsetof(nb::Ctl, dq));
@@ -190,12 +191,12 @@ int launch_force_method(nb_sim *s, int method, int fmode)
     switch (method) {
     case NB_MODE_PLAIN: e = nb::launch_force(d, s->p.vote_width, fmode, s->stream); break;
     case NB_MODE_COARSE:
-        e = nb::launch_coarse(d, s->L, nb::kOutFinish, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
+        e = nb::launch_coarse(d, s->L, nb::kOutFinish, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
                               fmode, s->stream);
         break;
     case NB_MODE_LEARNED:
         if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
-        e = nb::launch_coarse(d, s->L, nb::kOutLearned, dq_coarse_ptr(s), nullptr, 0, nullptr, nullptr, nullptr,
+        e = nb::launch_coarse(d, s->L, nb::kOutLearned, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
                               fmode, s->stream);
         break;
     default: return fail(s, NB_E_ARG, "unknown force method %d", method);
@@ -244,6 +245,7 @@ int ensure_quad(nb_sim *s)
     if (!d.cq64) HIP_TRY(s, dalloc(&d.cq64, 8 * cap));
     if (!d.cqa) HIP_TRY(s, dalloc(&d.cqa, 8 * cap));
     if (!d.cqb) HIP_TRY(s, dalloc(&d.cqb, 8 * cap));
+    if (!d.cmac) HIP_TRY(s, dalloc(&d.cmac, 8 * cap));
     return NB_OK;
 }
 
@@ -337,7 +339,7 @@ void free_all(nb_sim *s)
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
-                    s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb,
+                    s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
                     s->mlpbuf, s->chk_acc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -478,7 +480,8 @@ void nb_default_params(nb_params *p, int32_t n_bodies)
     p->device = 0;
     p->pool_factor = 2.0f;      // NUMBER_OF_NODES = 2 * NBODIES (java:220)
     p->mode = NB_MODE_PLAIN;
-    p->coarse_theta = 0.9f;
+    p->coarse_theta = 0.95f;
+    p->coarse_alpha = 0.02f;
     p->check_tol = 1e-3f;
     p->check_every = 0;
     p->check_samples = 4096;
@@ -519,7 +522,12 @@ int nb_create(const nb_params *p, nb_sim **out)
     d.theta2 = p->theta * p->theta;       // THETA = theta^2
     d.eps2 = p->softening * p->softening; // EPSILON = softening^2
     d.dt = p->dt;
-    d.theta2_coarse = p->coarse_theta * p->coarse_theta;
+    {
+        // the oracle derives the same two FP32 constants in the same way
+        const float theta2c = p->coarse_theta * p->coarse_theta;
+        d.it2_coarse = 1.0f / theta2c;
+        d.alpha2_coarse = p->coarse_alpha > 0.0f ? (float)((double)p->coarse_alpha * (double)p->coarse_alpha) : 0.0f;
+    }
     d.fp64 = (p->flags & NB_FP64_POSITIONS) ? 1 : 0;
     d.quad = p->mode != NB_MODE_PLAIN ? 1 : 0;
     const size_t cap = (size_t)d.cap;
@@ -1039,7 +1047,7 @@ int nb_train(nb_sim *s, const nb_train_params *tp_in, nb_train_report *rep)
             r = launch_stage(s, st);
             if (r) return r;
         }
-        hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, dq_coarse_ptr(s), gdev + (size_t)k * G, G,
+        hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, nullptr, gdev + (size_t)k * G, G,
                                          X + (size_t)k * rows * nb::kFeat, frame, nullptr, 0, s->stream);
         if (e == hipSuccess)
             e = nb::launch_coarse(d, s->L, nb::kOutAccel, dq_plain_ptr(s), gdev + (size_t)k * G, G, nullptr,
@@ -1138,7 +1146,7 @@ int nb_coarse_features(nb_sim *s, float *feat, float *acc3, float *e2)
         r = launch_stage(s, st);
         if (r) return r;
     }
-    hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, dq_coarse_ptr(s), nullptr, 0, xr, fr, nullptr, 0, s->stream);
+    hipError_t e = nb::launch_coarse(d, s->L, nb::kOutRaw, nullptr, nullptr, 0, xr, fr, nullptr, 0, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "coarse launch: %s", hipGetErrorString(e));
     HIP_TRY(s, hipMemcpyAsync(&d.ctl->step, &c0.step, sizeof(int), hipMemcpyHostToDevice, s->stream));
     r = check_device_error(s);

@@ -67,7 +67,6 @@ struct Ctl {
     float radius;        // reference _radius
     float root[3];
     float dq[kMaxDepth]; // opening thresholds (calculateforce.cl:52-67)
-    float dq_coarse[kMaxDepth]; // the same for the coarse opening angle (learned mode)
     unsigned long long interactions; // cumulative body-node interactions
     int levels;          // number of non-empty cell levels of the last tree
     int live_cells;      // cells linked into the tree by the last build
@@ -79,7 +78,8 @@ struct Dev {
     int n;
     int cap; // cells in the pool
     float theta2, eps2, dt;
-    float theta2_coarse; // learned mode: coarse opening angle squared
+    float it2_coarse;    // learned mode: 1 / coarse_theta^2 (the extent test)
+    float alpha2_coarse; // learned mode: coarse_alpha^2 (the relative test; 0 = off)
     int fp64;
     int quad;            // summarize also computes quadrupole moments
     Ctl *ctl;
@@ -94,6 +94,7 @@ struct Dev {
     double *cq64;
     float4 *cqa;
     float4 *cqb;
+    float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^2) of the child's extent b
     int *lvl_list;
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
     float4 *bnode;
@@ -162,6 +163,9 @@ struct Learn {
 // features and frames of listed groups (training pairs), accelerations of
 // listed groups (training targets), or the body update with the coarse
 // acceleration alone.
+// kOutAccel opens cells with the force stage's table `dqtab` (the fine
+// traversal of the training targets); the others with the extent and
+// relative tests of the coarse traversal (dqtab unused).
 enum CoarseOut { kOutLearned = 0, kOutRaw = 1, kOutAccel = 2, kOutFinish = 3 };
 hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,
                          int ngroups, float *xraw, float4 *frame_out, float4 *acc_out, int fmode,

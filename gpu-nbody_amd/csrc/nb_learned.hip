@@ -1,32 +1,44 @@
 // nb_learned.hip -- the learned coarse-theta correction for gfx950.
 //
 // Coarse traversal. The wave-uniform traversal of the force stage (one wave
-// = one vote group of 64 sorted bodies) at a coarser opening angle. Every
-// accepted cell contributes its monopole and its quadrupole under the
-// softened kernel: with d = centre of mass - body, s^2 = |d|^2 + eps^2 and the
-// cell's traceless second moment Q and trace T about its centre of mass,
+// = one vote group of 64 sorted bodies, a cell is taken when all 64 lanes
+// accept it) with a coarser test. A lane accepts a cell of extent b (the
+// largest distance of its bodies from its centre of mass, bounded in
+// summarize) and mass m at s^2 = |d|^2 + eps^2 when
+//     s^2 >= b^2 / theta_c^2 + eps^2           (extent test: |d| >= b / theta_c)
+//     (m b^3)^2 <= alpha^2 |a_old|^2 s^10      (m b^3 / s^5, the size of the
+//                                               first term left out, within
+//                                               alpha |a_old|)
+// with a_old the body's acceleration of the previous step (the relative test
+// is off for a lane while there is none). Every accepted cell contributes its
+// monopole and its quadrupole under the softened kernel: with d = centre of
+// mass - body and the cell's traceless second moment Q and trace T about its
+// centre of mass,
 //     a = m d / s^3 - Q d / s^5 + 5/2 (d.Q.d - T eps^2) d / s^7 .
 // (The T eps^2 term is the part of the second-order expansion of the
 // softened kernel that the traceless Q does not carry; it matters once cells
-// sit within a few softening lengths.) Bodies interact as point masses.
+// sit within a few softening lengths.) Bodies interact as point masses. The
+// training targets use the same kernel with the force stage's opening table
+// (kOutAccel).
 //
 // Features. Per body, 24 numbers in a frame built from the body's own coarse
 // acceleration a: e1 = a / |a|, e2 = the part of the quadrupole acceleration
 // a_q perpendicular to e1, normalized, e3 = e1 x e2. Vectors are projected on
 // the frame and divided by |a|, so the features do not change under rotations
-// and under a common scaling of the masses.
+// and under a common scaling of the masses. Only four sums per accepted cell;
+// the rest comes once per body from wave reductions.
 //    0-2  acceleration from bodies (a_b)
 //    3-4  a_q along e1, e2 (its e3 component is 0 by construction)
-//    5-7  sum over accepted cells of u * (cell monopole), u = dq[depth] / s^2
-//         in (0, 1]: how close the cell came to being opened
-//    8-10 the same weighted by u^2
-//   11-12 log(1 + sum u), log(1 + sum u^2)
-//   13-14 log(1 + accepted cells), log(1 + bodies met)
-//   15-16 log(|a_q| / |a|), log(|a_b| / |a|)
-//   17    mass-weighted mean u of the accepted cells
-//   18    accepted-cell mass / (accepted-cell mass + body mass met)
-//   19-21 offset of the body from its group's centroid in the frame, over the
-//         group radius
+//    5-7  a minus the mean coarse acceleration of the group
+//    8-10 offset of the body from its group's centroid, over the group radius
+//   11-13 the group's mean a_b
+//   14-15 log(1 + sum u), log(1 + sum u^2) over accepted cells, with
+//         u = (b^2 / theta_c^2 + eps^2) / s^2 in (0, 1]: how close the cell
+//         came to failing the extent test
+//   16-17 log(1 + accepted cells), log(1 + bodies met)
+//   18-19 log(|a_q| / |a|), log(|a_b| / |a|)
+//   20    mass-weighted mean u of the accepted cells
+//   21    accepted-cell mass / (accepted-cell mass + body mass met)
 //   22    log(group radius / root radius)
 //   23    cosine between the cell monopole part and a_b
 //
@@ -170,12 +182,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                                                   const float4 *__restrict__ cdata,
                                                   const float4 *__restrict__ cqa,
                                                   const float4 *__restrict__ cqb,
+                                                  const float2 *__restrict__ cmac,
                                                   const float *__restrict__ dqtab,
                                                   const int *__restrict__ groups, int ngroups,
                                                   float *__restrict__ xraw, float4 *__restrict__ frame_out,
                                                   float4 *__restrict__ acc_out, int fmode)
 {
     constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
+    constexpr bool kGeo = OUT == kOutAccel; // the fine traversal: the force stage's opening table
     // a wave's traversal stack; after the traversal the learned step reuses it
     // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
     __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
@@ -190,7 +204,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         if (tid == 0) flag_error(ctl, kErrDepth);
         return;
     }
-    if (tid < kMaxDepth) dq_s[tid] = dqtab[tid];
+    if (kGeo && tid < kMaxDepth) dq_s[tid] = dqtab[tid];
     const int step = ctl->step;
     const float root_r = ctl->radius;
     __syncthreads();
@@ -209,13 +223,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
+    // alpha^2 |a_old|^2 of the relative test; +inf switches it off (no
+    // previous acceleration yet, as in the force stage's kick, or alpha = 0)
+    float aa = __builtin_inff();
+    if (!kGeo && valid && step > 0 && d.alpha2_coarse > 0.f) {
+        const float4 a0 = d.acc4[d.sorted[pos]];
+        const float a2 = fmaf(a0.x, a0.x, fmaf(a0.y, a0.y, a0.z * a0.z));
+        if (a2 > 0.f) aa = d.alpha2_coarse * a2;
+    }
 
     if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
     int sp = 1;
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f; // total, Kahan
     float bx = 0.f, by = 0.f, bz = 0.f;                                // bodies
     float qx = 0.f, qy = 0.f, qz = 0.f;                                // quadrupole part
-    float ux = 0.f, uy = 0.f, uz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;  // u, u^2 weighted monopoles
     float su = 0.f, sv = 0.f, smu = 0.f, smc = 0.f, smb = 0.f;
     int nc = 0, nbod = 0;
     long long ninter = 0;
@@ -226,15 +247,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         const unsigned e = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
         const int c = (int)(e >> 6);
         const int dep = (int)(e & 63u);
-        const float thr = dq_s[dep];
+        const float thr_geo = kGeo ? dq_s[dep] : 0.f;
         const int *chp = cchild + (size_t)c * 8;
         const float4 *cdp = cdata + (size_t)c * 8;
         int ch[8];
         float4 q[8];
+        float2 mc[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             ch[k] = __builtin_amdgcn_readfirstlane(chp[k]);
             q[k] = cdp[k];
+            if (!kGeo) mc[k] = cmac[(size_t)c * 8 + k];
         }
         // per popped cell: bodies, cell monopoles and cell quadrupoles summed
         // separately (the features need the parts), then one compensated add
@@ -258,7 +281,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     ++nbod;
                     smb += q[k].w;
                 }
-            } else if (wave_all(d2 >= thr)) {
+                continue;
+            }
+            // a cell: the opening test of this traversal (for the coarse one,
+            // both conditions evaluated without a short circuit, so no lane
+            // branches; the threshold b^2 / theta_c^2 + eps^2 comes from summarize)
+            const float thr = kGeo ? thr_geo : mc[k].x;
+            float p = d2 * d2;
+            p = p * p;
+            p = p * d2;
+            // lane masks straight from the compares (3 = ordered >=, 5 = ordered <=)
+            const uint64_t ok = kGeo ? __builtin_amdgcn_fcmpf(d2, thr, 3)
+                                     : __builtin_amdgcn_fcmpf(d2, thr, 3) & __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5);
+            if (ok == __builtin_amdgcn_read_exec()) {
                 const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qzz, T eps^2)
                 const float4 qb = cqb[(size_t)c * 8 + k]; // (Qxy, Qxz, Qyz, 0)
                 const float ri = __builtin_amdgcn_rsqf(d2);
@@ -280,14 +315,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
                     pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
                     const float u = thr * r2i;
-                    const float fu = f * u;
-                    ux = fmaf(fu, dx, ux);
-                    uy = fmaf(fu, dy, uy);
-                    uz = fmaf(fu, dz, uz);
-                    const float fu2 = fu * u;
-                    vx = fmaf(fu2, dx, vx);
-                    vy = fmaf(fu2, dy, vy);
-                    vz = fmaf(fu2, dz, vz);
                     su += u;
                     sv = fmaf(u, u, sv);
                     smu = fmaf(u, q[k].w, smu);
@@ -332,18 +359,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     } else if constexpr (OUT == kOutFinish) {
         if (valid) finish_body(d, pos, me, ax, ay, az, fmode, step);
     } else {
-        // group centroid and radius (wave reductions over the valid lanes)
+        // group centroid and radius, mean acceleration and mean body part
+        // (wave reductions over the valid lanes)
         const float nv = (float)__popcll(__ballot(valid));
         float gx = valid ? me.x : 0.f, gy = valid ? me.y : 0.f, gz = valid ? me.z : 0.f;
+        float gax = valid ? ax : 0.f, gay = valid ? ay : 0.f, gaz = valid ? az : 0.f;
+        float gbx = valid ? bx : 0.f, gby = valid ? by : 0.f, gbz = valid ? bz : 0.f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             gx += __shfl_xor(gx, o);
             gy += __shfl_xor(gy, o);
             gz += __shfl_xor(gz, o);
+            gax += __shfl_xor(gax, o);
+            gay += __shfl_xor(gay, o);
+            gaz += __shfl_xor(gaz, o);
+            gbx += __shfl_xor(gbx, o);
+            gby += __shfl_xor(gby, o);
+            gbz += __shfl_xor(gbz, o);
         }
         gx /= nv;
         gy /= nv;
         gz /= nv;
+        gax /= nv;
+        gay /= nv;
+        gaz /= nv;
+        gbx /= nv;
+        gby /= nv;
+        gbz /= nv;
         const float ox = me.x - gx, oy = me.y - gy, oz = me.z - gz;
         float rg = valid ? sqrtf(ox * ox + oy * oy + oz * oz) : 0.f;
 #pragma unroll
@@ -385,24 +427,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             NB_LOC(bx, by, bz, 0);
             f[3] = qe1 * inv;
             f[4] = (qx * e2x + qy * e2y + qz * e2z) * inv;
-            NB_LOC(ux, uy, uz, 5);
-            NB_LOC(vx, vy, vz, 8);
-            f[11] = log1pf(su);
-            f[12] = log1pf(sv);
-            f[13] = log1pf((float)nc);
-            f[14] = log1pf((float)nbod);
-            const float qn = sqrtf(qx * qx + qy * qy + qz * qz);
-            const float bn = sqrtf(bx * bx + by * by + bz * bz);
-            f[15] = logf(fmaxf(qn * inv, 1e-30f));
-            f[16] = logf(fmaxf(bn * inv, 1e-30f));
-            f[17] = smc > 0.f ? smu / smc : 0.f;
-            f[18] = (smc + smb) > 0.f ? smc / (smc + smb) : 0.f;
+            NB_LOC(ax - gax, ay - gay, az - gaz, 5);
             if (rg > 0.f) {
                 const float ir = 1.0f / rg;
-                f[19] = (ox * e1x + oy * e1y + oz * e1z) * ir;
-                f[20] = (ox * e2x + oy * e2y + oz * e2z) * ir;
-                f[21] = (ox * e3x + oy * e3y + oz * e3z) * ir;
+                f[8] = (ox * e1x + oy * e1y + oz * e1z) * ir;
+                f[9] = (ox * e2x + oy * e2y + oz * e2z) * ir;
+                f[10] = (ox * e3x + oy * e3y + oz * e3z) * ir;
             }
+            NB_LOC(gbx, gby, gbz, 11);
+            f[14] = log1pf(su);
+            f[15] = log1pf(sv);
+            f[16] = log1pf((float)nc);
+            f[17] = log1pf((float)nbod);
+            const float qn = sqrtf(qx * qx + qy * qy + qz * qz);
+            const float bn = sqrtf(bx * bx + by * by + bz * bz);
+            f[18] = logf(fmaxf(qn * inv, 1e-30f));
+            f[19] = logf(fmaxf(bn * inv, 1e-30f));
+            f[20] = smc > 0.f ? smu / smc : 0.f;
+            f[21] = (smc + smb) > 0.f ? smc / (smc + smb) : 0.f;
             f[22] = logf(fmaxf(rg / root_r, 1e-30f));
             const float mx = ax - bx - qx, my = ay - by - qy, mz = az - bz - qz;
             const float mn = sqrtf(mx * mx + my * my + mz * mz);
@@ -465,7 +507,7 @@ hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqt
     const dim3 grid((waves + 3) / 4), block(256);
 #define NB_COARSE_ARGS                                                                             \
     d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float4 *)d.cqb, \
-        dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
+        (const float2 *)d.cmac, dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
     switch (out) {
     case kOutLearned: hipLaunchKernelGGL(k_coarse64<kOutLearned>, grid, block, 0, st, NB_COARSE_ARGS); break;
     case kOutRaw: hipLaunchKernelGGL(k_coarse64<kOutRaw>, grid, block, 0, st, NB_COARSE_ARGS); break;

@@ -136,14 +136,11 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
         c->radius = radius;
         // opening thresholds, calculateforce.cl:52-67 (FP32 quotient via a
         // double division rounded once = the correctly rounded FP32 quotient)
-        float raw = radius * radius, rawc = radius * radius;
+        float raw = radius * radius;
         if (d.theta2 > 0.0f) raw = (float)((double)raw / (double)d.theta2);
-        if (d.theta2_coarse > 0.0f) rawc = (float)((double)rawc / (double)d.theta2_coarse);
         for (int i = 0; i < kMaxDepth; ++i) {
             c->dq[i] = raw + d.eps2;
-            c->dq_coarse[i] = rawc + d.eps2;
             raw = 0.25f * raw;
-            rawc = 0.25f * rawc;
         }
         c->n_cells = 1; // the root is cell 0 (reference: _bottom = NUMBER_OF_NODES, :180)
         c->epoch += 1;  // every child slot of the last tree turns stale
@@ -476,6 +473,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     int cnt = 0;
     float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
     double q[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; // the child's own second moments
+    double bc = 0.0;                                     // and its extent
     if (v >= 0 && v < n) {
         rec = d.pm4[v];
         m = rec.w;
@@ -496,6 +494,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             const double4 q0 = cq[0], q1 = cq[1];
             q[0] = q0.x; q[1] = q0.y; q[2] = q0.z; q[3] = q0.w;
             q[4] = q1.x; q[5] = q1.y; q[6] = q1.z;
+            bc = q1.w;
         }
     }
     // compaction: position of this child among the cell's non-empty slots;
@@ -530,7 +529,18 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
         // over the slots in the same pairwise order. Each child's moments also
         // sit next to its (x, y, z, m) for the traversal, as
         // (Qxx, Qyy, Qzz, T eps^2) and (Qxy, Qxz, Qyz, 0).
+        // Extent b: the largest (|child centre - centre| + child extent) over
+        // the slots, an upper bound of every body's distance from the centre
+        // of mass (a body's extent is 0); each child's own b enters the
+        // coarse traversal's tests as the threshold b^2 / theta_c^2 + eps^2
+        // (an FP32 fmaf of b^2 rounded from FP64) and (m b^3)^2.
+        // Explicit fma and no add after a product: the opening decisions that
+        // follow from these values are bit-exact against the oracle.
         const double dx = x - com.x, dy = y - com.y, dz = z - com.z;
+        double br = v >= 0 ? sqrt(fma(dx, dx, fma(dy, dy, dz * dz))) + bc : 0.0;
+        br = fmax(br, __shfl_xor(br, 1));
+        br = fmax(br, __shfl_xor(br, 2));
+        br = fmax(br, __shfl_xor(br, 4));
         const double r2 = dx * dx + dy * dy + dz * dz;
         double t7[7];
         t7[0] = q[0] + m * (3.0 * dx * dx - r2);
@@ -546,10 +556,12 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
             const double eps2 = (double)d.eps2;
             d.cqa[(size_t)c * 8 + at] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[6] * eps2));
             d.cqb[(size_t)c * 8 + at] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
+            const double mb3 = m * ((bc * bc) * bc);
+            d.cmac[(size_t)c * 8 + at] = make_float2(fmaf((float)(bc * bc), d.it2_coarse, d.eps2), (float)(mb3 * mb3));
             if (k == 0) {
                 double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);
                 cq[0] = make_double4(t7[0], t7[1], t7[2], t7[3]);
-                cq[1] = make_double4(t7[4], t7[5], t7[6], 0.0);
+                cq[1] = make_double4(t7[4], t7[5], t7[6], br);
             }
         }
     }

@@ -53,7 +53,8 @@ class NbParams(C.Structure):
         ("check_tol", C.c_float),
         ("check_every", C.c_int32),
         ("check_samples", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("coarse_alpha", C.c_float),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

@@ -63,12 +63,16 @@ typedef struct nb_params {
     int32_t device;     /* HIP device ordinal */
     float pool_factor;  /* octree cell pool = max(pool_factor*N, 16384) cells */
     int32_t mode;         /* NB_MODE_PLAIN or NB_MODE_LEARNED */
-    float coarse_theta;   /* learned mode: opening angle of the coarse traversal */
+    float coarse_theta;   /* learned mode: opening angle of the coarse traversal's extent
+                             test |d| >= b / coarse_theta (b: the cell's extent); in
+                             [theta, 1) */
     float check_tol;      /* learned mode: median relative error allowed on the sampled
                              FP64 check before falling back to plain theta; 0 = no check */
     int32_t check_every;  /* learned mode: run the sampled check every k steps (0 = never) */
     int32_t check_samples;/* learned mode: bodies in the sampled check */
-    int32_t reserved[7];
+    float coarse_alpha;   /* learned mode: relative test m b^3 / s^5 <= coarse_alpha |a_old|
+                             of the coarse traversal (0 = off) */
+    int32_t reserved[6];
 } nb_params;
 
 /* Defaults are the reference's shipped constants: THETA 1.5, EPSILON 0.0025,
@@ -180,13 +184,17 @@ int nb_force_error(nb_sim *sim, int32_t nsamples, uint64_t seed, double *stats4)
 
 /* ---- learned coarse-theta correction (no reference counterpart) ----------
  *
- * The coarse traversal opens cells at `coarse_theta` and adds, to the
+ * The coarse traversal accepts a cell of extent b (the largest distance of its
+ * bodies from its centre of mass) and mass m at softened distance s when, for
+ * all 64 bodies of the vote group, |d| >= b / coarse_theta and
+ * m b^3 / s^5 <= coarse_alpha |a_old| (a_old: the body's acceleration of the
+ * previous step; this test is off until there is one), and adds, to the
  * monopole of every accepted cell, its quadrupole under the softened kernel.
  * It also emits 24 features per body in a frame built from the body's own
  * coarse acceleration a (e1 = a/|a|, e2 from the quadrupole part, e3 = e1 x e2).
  * A per-body MLP 24 -> 64 -> 64 -> 3 (tanh-GELU) maps the normalized features
  * to the residual (a_fine - a) in that frame over |a|, where a_fine is the
- * same traversal at `theta`. Inference runs with BF16 weights and activations
+ * same traversal with the force stage's opening table at `theta`. Inference runs with BF16 weights and activations
  * on the matrix cores with FP32 accumulation; training runs in FP32 with
  * Adam on pairs sampled from the simulation itself.
  *

@@ -315,13 +315,14 @@ void orc_summarize(orc_sim *s)
     const int fp32 = (s->flags & ORC_COM_FP32) != 0;
     if (!fp32 && !s->cq) s->cq = (double *)calloc(8 * ((size_t)nn + 1), sizeof(double));
     for (int node = s->bottom; node <= nn; ++node) {
-        int used = 0, cnt = 0;
-        double mk[8], xk[8], yk[8], zk[8], qk[8][7];
+        int used = 0, cnt = 0, has[8];
+        double mk[8], xk[8], yk[8], zk[8], qk[8][7], bk[8];
         float m32 = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (int k = 0; k < 8; ++k) {
             const int c = child[8 * node + k];
-            mk[k] = xk[k] = yk[k] = zk[k] = 0.0;
+            mk[k] = xk[k] = yk[k] = zk[k] = bk[k] = 0.0;
             for (int j = 0; j < 7; ++j) qk[k][j] = 0.0;
+            has[k] = c >= 0;
             if (c < 0) continue;
             if (k != used) { /* move to front, :77-81 */
                 child[8 * node + k] = -1;
@@ -338,6 +339,7 @@ void orc_summarize(orc_sim *s)
                     const double *cc = s->com + 4 * (size_t)c;
                     mk[k] = cc[3]; xk[k] = cc[0]; yk[k] = cc[1]; zk[k] = cc[2];
                     for (int j = 0; j < 7; ++j) qk[k][j] = s->cq[8 * (size_t)c + j];
+                    bk[k] = s->cq[8 * (size_t)c + 7];
                 }
             } else {
                 cnt += 1;
@@ -368,10 +370,15 @@ void orc_summarize(orc_sim *s)
             X[node] = (float)cc[0]; Y[node] = (float)cc[1]; Z[node] = (float)cc[2];
             M[node] = (float)m64; /* the ready flag of :172, as a value */
             /* second moments: per slot, the child's own plus its mass at its
-             * offset from this centre of mass (parallel-axis rule) */
-            double t[7][8];
+             * offset from this centre of mass (parallel-axis rule); extent:
+             * the largest offset plus the child's own extent */
+            double t[7][8], bmax = 0.0;
             for (int k = 0; k < 8; ++k) {
                 const double dx = xk[k] - cc[0], dy = yk[k] - cc[1], dz = zk[k] - cc[2];
+                if (has[k]) {
+                    const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz))) + bk[k];
+                    bmax = r > bmax ? r : bmax;
+                }
                 const double r2 = dx * dx + dy * dy + dz * dz, mx = mk[k];
                 t[0][k] = qk[k][0] + mx * (3.0 * dx * dx - r2);
                 t[1][k] = qk[k][1] + mx * (3.0 * dy * dy - r2);
@@ -382,7 +389,7 @@ void orc_summarize(orc_sim *s)
                 t[6][k] = qk[k][6] + mx * r2;
             }
             for (int j = 0; j < 7; ++j) s->cq[8 * (size_t)node + j] = pair8(t[j]);
-            s->cq[8 * (size_t)node + 7] = 0.0;
+            s->cq[8 * (size_t)node + 7] = bmax;
         }
     }
 }
@@ -852,14 +859,29 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
  * traceless Q = sum m (3 s s^T - |s|^2 I) (xx, yy, zz, xy, xz, yz) and trace
  * T = sum m |s|^2 about its FP64 centre of mass, moved to the parent by the
  * parallel-axis rule, summed over the octant slots in the pairwise order of
- * the centre of mass. Bodies are point masses at their FP32 positions.
+ * the centre of mass; and its extent b = max over children of
+ * (|child centre - centre| + child extent), an upper bound of the distance of
+ * any of its bodies from its centre of mass (bodies have extent 0). Bodies are
+ * point masses at their FP32 positions.
  *
  * orc_coarse: the wave-uniform traversal of the force stage, restated per vote
- * group with the opening table of `theta`: accepted cells contribute
+ * group: accepted cells contribute
  *     m d / s^3 - Q d / s^5 + 5/2 (d.Q.d - T eps^2) d / s^7
  * (d = centre of mass - body, s^2 = |d|^2 + eps^2, the second-order expansion
- * of the softened kernel), bodies m d / s^3. Opening decisions use the FP32
- * squared distance of the force stage; the terms are summed in FP64. For each
+ * of the softened kernel), bodies m d / s^3. A cell is accepted when every
+ * lane of the group accepts it. With mac = ORC_MAC_GEOMETRIC a lane accepts
+ * when s^2 >= dq[depth] (the opening table of `theta`, as in the force stage).
+ * With mac = ORC_MAC_EXTENT a lane accepts when both
+ *     s^2 >= b^2 / theta^2 + eps^2        (the extent test: |d| >= b / theta)
+ *     (m b^3)^2 <= alpha^2 |a_old|^2 s^10  (m b^3 / s^5, the size of the first
+ *                                           term left out, within alpha |a_old|)
+ * hold, with a_old the body's acceleration of the previous step (the test is
+ * skipped for a lane while step <= 0, when the force stage has no previous
+ * acceleration either (calculateforce.cl:174-179), when |a_old| = 0, or when
+ * alpha <= 0). Opening decisions are made in
+ * FP32 with the engine's operation order: s^2 = fmaf chain, thr = fmaf(b^2,
+ * 1/theta^2, eps^2), s^10 = ((s^2 s^2)(s^2 s^2)) s^2, with b^2 and (m b^3)^2
+ * rounded from FP64 per child; the force terms are summed in FP64. For each
  * body of the listed groups it writes the acceleration (acc, 3 per body) and,
  * when feat is not NULL, the 24 features of the engine's coarse kernel and the
  * frame vector e2 (3 per body). Output row of lane l of the i-th listed group:
@@ -870,15 +892,35 @@ void orc_moments(orc_sim *s)
     (void)s;
 }
 
-/* second moments of a cell (for tests): out7 = Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, T */
-int orc_get_moments(const orc_sim *s, int cell, double *out7)
+/* second moments of a cell (for tests): out8 = Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, T, extent */
+int orc_get_moments(const orc_sim *s, int cell, double *out8)
 {
     if (!s->cq || cell < s->n || cell > s->nn) return -1;
-    for (int j = 0; j < 7; ++j) out7[j] = s->cq[8 * (size_t)cell + j];
+    for (int j = 0; j < 8; ++j) out8[j] = s->cq[8 * (size_t)cell + j];
     return 0;
 }
 
-static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, float *feat, double *acc,
+#define ORC_MAC_GEOMETRIC 0
+#define ORC_MAC_EXTENT 1
+
+typedef struct coarse_cfg {
+    int mac;
+    const float *dq; /* geometric: the opening table */
+    float it2;       /* extent: 1 / theta^2 */
+    float alpha2;    /* extent: alpha^2 (<= 0: no relative test) */
+} coarse_cfg;
+
+/* per child: the threshold fmaf(b^2, 1/theta^2, eps^2) of b^2 rounded to FP32
+ * from the FP64 extent, and (m b^3)^2 rounded from FP64 */
+static void extent_terms(const orc_sim *s, int c, float it2, float *thr, float *mb3sq)
+{
+    const double b = s->cq[8 * (size_t)c + 7], mc = s->com[4 * (size_t)c + 3];
+    const double mb3 = mc * ((b * b) * b);
+    *thr = fmaf((float)(b * b), it2, s->eps2);
+    *mb3sq = (float)(mb3 * mb3);
+}
+
+static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slot0, float *feat, double *acc,
                          double *e2out, long long *inter)
 {
     const int n = s->n, nn = s->nn, W = s->vote;
@@ -888,17 +930,19 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
     const int first = g * W;
     const int cnt = (first + W <= n) ? W : n - first;
     if (cnt <= 0) return;
-    float px[256], py[256], pz[256];
-    /* per lane: total, bodies, quadrupole part, u- and u^2-weighted cell monopoles */
-    double A[256][3], Bd[256][3], Qp[256][3], U1[256][3], U2[256][3], su[256], sv[256], smu[256];
+    float px[256], py[256], pz[256], aa[256];
+    /* per lane: total, bodies, quadrupole part; u sums */
+    double A[256][3], Bd[256][3], Qp[256][3], su[256], sv[256], smu[256];
     double smc = 0.0, smb = 0.0;
     long long nc = 0, nbod = 0;
     int stackNode[ORC_MAXDEPTH + 1], stackPos[ORC_MAXDEPTH + 1];
     for (int l = 0; l < cnt; ++l) {
         const int b = s->sorted[first + l];
         px[l] = X[b]; py[l] = Y[b]; pz[l] = Z[b];
-        for (int j = 0; j < 3; ++j) A[l][j] = Bd[l][j] = Qp[l][j] = U1[l][j] = U2[l][j] = 0.0;
+        for (int j = 0; j < 3; ++j) A[l][j] = Bd[l][j] = Qp[l][j] = 0.0;
         su[l] = sv[l] = smu[l] = 0.0;
+        const float a2 = fmaf(s->ax[b], s->ax[b], fmaf(s->ay[b], s->ay[b], s->az[b] * s->az[b]));
+        aa[l] = (cfg->alpha2 > 0.0f && s->step > 0 && a2 > 0.0f) ? cfg->alpha2 * a2 : INFINITY;
     }
     long long ni = 0;
     int depth = 0;
@@ -915,11 +959,24 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
             }
             const float cxn = X[c], cyn = Y[c], czn = Z[c];
             int accept = 1;
+            float thr = 0.0f;
             if (c >= n) {
+                float mb3sq = 0.0f;
+                if (cfg->mac == ORC_MAC_EXTENT) {
+                    extent_terms(s, c, cfg->it2, &thr, &mb3sq);
+                } else {
+                    thr = cfg->dq[depth];
+                }
                 for (int l = 0; l < cnt && accept; ++l) {
                     const float dx = cxn - px[l], dy = cyn - py[l], dz = czn - pz[l];
                     const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-                    if (!(d2 >= dq[depth])) accept = 0;
+                    if (!(d2 >= thr)) accept = 0;
+                    if (accept && cfg->mac == ORC_MAC_EXTENT) {
+                        float p = d2 * d2;
+                        p = p * p;
+                        p = p * d2;
+                        if (!(mb3sq <= aa[l] * p)) accept = 0;
+                    }
                 }
             }
             if (!accept) {
@@ -951,9 +1008,7 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
                 const double gx = c2 * dx - r5 * Qx, gy = c2 * dy - r5 * Qy, gz = c2 * dz - r5 * Qz;
                 A[l][0] += gx; A[l][1] += gy; A[l][2] += gz;
                 Qp[l][0] += gx; Qp[l][1] += gy; Qp[l][2] += gz;
-                const double u = (double)dq[depth] * r2i;
-                U1[l][0] += u * f * dx; U1[l][1] += u * f * dy; U1[l][2] += u * f * dz;
-                U2[l][0] += u * u * f * dx; U2[l][1] += u * u * f * dy; U2[l][2] += u * u * f * dz;
+                const double u = (double)thr * r2i;
                 su[l] += u;
                 sv[l] += u * u;
                 smu[l] += u * mc;
@@ -963,10 +1018,14 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
         depth--;
     }
     *inter += ni;
-    /* group centroid and radius */
-    double gx = 0, gy = 0, gz = 0, rg = 0;
-    for (int l = 0; l < cnt; ++l) { gx += px[l]; gy += py[l]; gz += pz[l]; }
+    /* group centroid and radius, mean acceleration and mean body part */
+    double gx = 0, gy = 0, gz = 0, rg = 0, am[3] = {0, 0, 0}, bm[3] = {0, 0, 0};
+    for (int l = 0; l < cnt; ++l) {
+        gx += px[l]; gy += py[l]; gz += pz[l];
+        for (int j = 0; j < 3; ++j) { am[j] += A[l][j]; bm[j] += Bd[l][j]; }
+    }
     gx /= cnt; gy /= cnt; gz /= cnt;
+    for (int j = 0; j < 3; ++j) { am[j] /= cnt; bm[j] /= cnt; }
     for (int l = 0; l < cnt; ++l) {
         const double ox = px[l] - gx, oy = py[l] - gy, oz = pz[l] - gz;
         const double r = sqrt(ox * ox + oy * oy + oz * oz);
@@ -1001,33 +1060,32 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
             e3[0] = e1[1] * e2[2] - e1[2] * e2[1];
             e3[1] = e1[2] * e2[0] - e1[0] * e2[2];
             e3[2] = e1[0] * e2[1] - e1[1] * e2[0];
-#define ORC_LOC(V, I)                                                                              \
+#define ORC_LOC(V, I, S)                                                                           \
     do {                                                                                           \
-        f[I] = (float)((V[0] * e1[0] + V[1] * e1[1] + V[2] * e1[2]) * inv);                        \
-        f[(I) + 1] = (float)((V[0] * e2[0] + V[1] * e2[1] + V[2] * e2[2]) * inv);                  \
-        f[(I) + 2] = (float)((V[0] * e3[0] + V[1] * e3[1] + V[2] * e3[2]) * inv);                  \
+        f[I] = (float)((V[0] * e1[0] + V[1] * e1[1] + V[2] * e1[2]) * (S));                        \
+        f[(I) + 1] = (float)((V[0] * e2[0] + V[1] * e2[1] + V[2] * e2[2]) * (S));                  \
+        f[(I) + 2] = (float)((V[0] * e3[0] + V[1] * e3[1] + V[2] * e3[2]) * (S));                  \
     } while (0)
-            ORC_LOC(Bd[l], 0);
+            ORC_LOC(Bd[l], 0, inv);
             f[3] = (float)(qe1 * inv);
             f[4] = (float)((qv[0] * e2[0] + qv[1] * e2[1] + qv[2] * e2[2]) * inv);
-            ORC_LOC(U1[l], 5);
-            ORC_LOC(U2[l], 8);
-            f[11] = (float)log1p(su[l]);
-            f[12] = (float)log1p(sv[l]);
-            f[13] = (float)log1p((double)nc);
-            f[14] = (float)log1p((double)nbod);
-            const double qn = sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]);
-            const double bn = sqrt(Bd[l][0] * Bd[l][0] + Bd[l][1] * Bd[l][1] + Bd[l][2] * Bd[l][2]);
-            f[15] = (float)log(fmax(qn * inv, 1e-30));
-            f[16] = (float)log(fmax(bn * inv, 1e-30));
-            f[17] = smc > 0.0 ? (float)(smu[l] / smc) : 0.0f;
-            f[18] = (smc + smb) > 0.0 ? (float)(smc / (smc + smb)) : 0.0f;
+            const double tv[3] = {a[0] - am[0], a[1] - am[1], a[2] - am[2]};
+            ORC_LOC(tv, 5, inv);
             if (rg > 0.0) {
                 const double o3[3] = {px[l] - gx, py[l] - gy, pz[l] - gz};
-                f[19] = (float)((o3[0] * e1[0] + o3[1] * e1[1] + o3[2] * e1[2]) / rg);
-                f[20] = (float)((o3[0] * e2[0] + o3[1] * e2[1] + o3[2] * e2[2]) / rg);
-                f[21] = (float)((o3[0] * e3[0] + o3[1] * e3[1] + o3[2] * e3[2]) / rg);
+                ORC_LOC(o3, 8, 1.0 / rg);
             }
+            ORC_LOC(bm, 11, inv);
+            f[14] = (float)log1p(su[l]);
+            f[15] = (float)log1p(sv[l]);
+            f[16] = (float)log1p((double)nc);
+            f[17] = (float)log1p((double)nbod);
+            const double qn = sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]);
+            const double bn = sqrt(Bd[l][0] * Bd[l][0] + Bd[l][1] * Bd[l][1] + Bd[l][2] * Bd[l][2]);
+            f[18] = (float)log(fmax(qn * inv, 1e-30));
+            f[19] = (float)log(fmax(bn * inv, 1e-30));
+            f[20] = smc > 0.0 ? (float)(smu[l] / smc) : 0.0f;
+            f[21] = (smc + smb) > 0.0 ? (float)(smc / (smc + smb)) : 0.0f;
             f[22] = (float)log(fmax(rg / (double)s->radius, 1e-30));
             const double mv[3] = {a[0] - Bd[l][0] - qv[0], a[1] - Bd[l][1] - qv[1], a[2] - Bd[l][2] - qv[2]};
             const double mn = sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
@@ -1040,14 +1098,21 @@ static void coarse_group(const orc_sim *s, const float *dq, int g, int slot0, fl
     }
 }
 
-long long orc_coarse(orc_sim *s, float theta, const int *groups, int ngroups, float *feat, double *acc,
-                     double *e2)
+long long orc_coarse(orc_sim *s, int mac, float theta, float alpha, const int *groups, int ngroups, float *feat,
+                     double *acc, double *e2)
 {
     float dq[ORC_MAXDEPTH];
     if (!s->cq || s->vote > 256 || s->max_depth > ORC_MAXDEPTH) return -1;
-    orc_dq_table(s->radius, theta * theta, s->eps2, s->max_depth, dq);
+    if (mac != ORC_MAC_GEOMETRIC && mac != ORC_MAC_EXTENT) return -1;
+    coarse_cfg cfg;
+    const float theta2 = theta * theta;
+    cfg.mac = mac;
+    cfg.dq = dq;
+    cfg.it2 = 1.0f / theta2;
+    cfg.alpha2 = alpha > 0.0f ? (float)((double)alpha * (double)alpha) : 0.0f;
+    orc_dq_table(s->radius, theta2, s->eps2, s->max_depth, dq);
     long long total = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
-    for (int i = 0; i < ngroups; ++i) coarse_group(s, dq, groups[i], i * s->vote, feat, acc, e2, &total);
+    for (int i = 0; i < ngroups; ++i) coarse_group(s, &cfg, groups[i], i * s->vote, feat, acc, e2, &total);
     return total;
 }

@@ -79,7 +79,7 @@ def lib():
         L.orc_moments.argtypes = [vp]
         L.orc_get_moments.argtypes = [vp, C.c_int, _f64p]
         L.orc_get_moments.restype = C.c_int
-        L.orc_coarse.argtypes = [vp, C.c_float, _i32p, C.c_int, vp, _f64p, vp]
+        L.orc_coarse.argtypes = [vp, C.c_int, C.c_float, C.c_float, _i32p, C.c_int, vp, _f64p, vp]
         L.orc_coarse.restype = C.c_longlong
         L.orc_mlp_params.restype = C.c_int
         L.orc_mlp_forward.argtypes = [_f64p, _f32p, C.c_int, _f64p]
@@ -240,16 +240,20 @@ class Oracle:
         lib().orc_moments(self.h)
 
     def cell_moments(self, cell):
-        out = np.zeros(7, np.float64)
+        """(Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, T, extent) of a cell (after stage 3)."""
+        out = np.zeros(8, np.float64)
         if lib().orc_get_moments(self.h, int(cell), out) != 0:
             raise ValueError("no moments for that node")
         return out
 
-    def coarse(self, theta, groups=None, features=True):
-        """Coarse traversal (monopole + softened quadrupole) at `theta` over the
-        listed vote groups (all groups in order when None): returns (acc [rows, 3],
-        features [rows, 24] or None, e2 [rows, 3] or None, interactions); row
-        i * vote + l is lane l of the i-th group (sorted position when all)."""
+    def coarse(self, theta, groups=None, features=True, alpha=0.0, geometric=False):
+        """Coarse traversal (monopole + softened quadrupole) over the listed vote
+        groups (all groups in order when None). geometric=False: the extent test
+        at `theta` and the relative test at `alpha` (the learned mode's coarse
+        traversal); geometric=True: the force stage's opening table of `theta`.
+        Returns (acc [rows, 3], features [rows, 24] or None, e2 [rows, 3] or
+        None, interactions); row i * vote + l is lane l of the i-th group (sorted
+        position when all)."""
         if groups is None:
             groups = np.arange((self.n + self.vote - 1) // self.vote, dtype=np.int32)
         g = np.ascontiguousarray(groups, dtype=np.int32)
@@ -257,7 +261,7 @@ class Oracle:
         acc = np.zeros((rows, 3), np.float64)
         feat = np.zeros((rows, 24), np.float32) if features else None
         e2 = np.zeros((rows, 3), np.float64) if features else None
-        ni = lib().orc_coarse(self.h, float(theta), g, len(g),
+        ni = lib().orc_coarse(self.h, 0 if geometric else 1, float(theta), float(alpha), g, len(g),
                               None if feat is None else feat.ctypes.data_as(C.c_void_p), acc.reshape(-1),
                               None if e2 is None else e2.ctypes.data_as(C.c_void_p))
         if ni < 0:

@@ -4,13 +4,23 @@ The coarse traversal, its features and the network have no reference
 counterpart, so the oracle is checked against what the mathematics fixes:
   moments     the root's second moments equal a brute-force sum over all
               bodies about the global centre of mass (parallel-axis chain)
+  extent      the root's extent bounds the distance of every body from the
+              centre of mass (brute force), and equals it where every child of
+              the cell is a body (closed form)
   quadrupole  a distant cluster seen through one accepted cell: the error of
               monopole + quadrupole falls as R^-3 while the monopole's falls as
               R^-2, also when the softening is comparable to R (which needs the
               trace term of the softened expansion); and the coarse traversal
               converges to the FP64 direct sum much faster than the monopole
-  features    invariance under scaling all masses by 2 (bit-exact), and the
-              identity f3^2 + f4^2 = (|a_q| / |a|)^2 = exp(2 f15), f4 >= 0
+  coarse test the relative test is inert before there is a previous
+              acceleration (bit-identical with and without it); with alpha -> 0
+              it opens down to the bodies (the FP64 direct sum), and a finite
+              alpha adds interactions and lowers the error
+  features    invariance under scaling all masses (and the previous
+              accelerations) by 2 (bit-exact); the identities
+              f3^2 + f4^2 = (|a_q| / |a|)^2 = exp(2 f18), f4 >= 0; and the group
+              sums that vanish: sum |a| (f5 e1 + f6 e2 + f7 e3) = sum (a - mean a)
+              and sum (f8 e1 + f9 e2 + f10 e3) = sum (x - centroid) / r_g
   network     wiring by one-path networks (a transposed or misindexed weight
               moves the output), GELU at known points, and the backward pass
               against central finite differences of the forward pass
@@ -36,7 +46,7 @@ def test_root_moments_brute_force(oracle):
     st = U.plummer(3000, seed=3)
     o = _tree(oracle, st)
     root = o.n and oracle.nodes_for(o.n)  # the root is node NN
-    q = o.cell_moments(root)
+    q = o.cell_moments(root)[:7]
     x, y, z, m = (np.asarray(a, np.float64) for a in (st[0], st[1], st[2], st[6]))
     M = m.sum()
     c = np.array([(m * x).sum(), (m * y).sum(), (m * z).sum()]) / M
@@ -75,7 +85,7 @@ def _probe_error(oracle, R, eps):
     n = len(st[0])
     o = _tree(oracle, st, theta=1.2, softening=eps, vote=1)
     srt = o.sorted()
-    acc, _, _, _ = o.coarse(1.2, features=False)   # monopole + quadrupole (vote 1: per body)
+    acc, _, _, _ = o.coarse(1.2, features=False, geometric=True)   # monopole + quadrupole (vote 1: per body)
     pos = int(np.where(srt == k)[0][0])            # the probe's sorted position
     a_quad = acc[pos]
     # the monopole alone: the plain force stage of the oracle with the same table
@@ -107,33 +117,116 @@ def test_coarse_traversal_converges_faster_than_monopole(oracle):
     srt = o.sorted()
     ref = oracle.direct_fp64(st[0], st[1], st[2], st[6], 0.01, srt)
     med = lambda a: np.median(np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1))
-    acc, _, _, _ = o.coarse(0.3, features=False)
+    acc, _, _, _ = o.coarse(0.3, features=False, geometric=True)
     o.run_stages(5, 5)
     mono = o.accel()[srt].astype(np.float64)
     assert med(acc) < 0.25 * med(mono), (med(acc), med(mono))
+
+
+def test_extent_bounds_bodies_and_closed_forms(oracle):
+    from nbody_amd import universe as U
+
+    # brute force: the root's extent bounds every body's distance from the
+    # centre of mass, and is within the triangle-inequality slack of it
+    st = U.plummer(3000, seed=3)
+    o = _tree(oracle, st)
+    root = oracle.nodes_for(o.n)
+    b = o.cell_moments(root)[7]
+    x, y, z, m = (np.asarray(a, np.float64) for a in (st[0], st[1], st[2], st[6]))
+    c = np.array([(m * x).sum(), (m * y).sum(), (m * z).sum()]) / m.sum()
+    rmax = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2).max()
+    assert rmax * (1 - 1e-12) <= b <= 3.0 * rmax, (b, rmax)
+    # closed form: one body in each octant of the root (every child is a body)
+    pts = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+    pts = pts * np.array([0.3, 0.5, 0.7]) + np.array([0.01, -0.02, 0.03]) * np.arange(8)[:, None]
+    mm = np.linspace(1.0, 2.0, 8)
+    st8 = tuple(np.ascontiguousarray(a, np.float32) for a in (pts[:, 0], pts[:, 1], pts[:, 2], np.zeros(8),
+                                                              np.zeros(8), np.zeros(8), mm))
+    o8 = _tree(oracle, st8)
+    p32 = np.stack(st8[:3], 1).astype(np.float64)
+    m32 = st8[6].astype(np.float64)
+    cell = oracle.nodes_for(8)
+    # the root's centre of mass as the oracle sums it (pairwise over the slots)
+    com = o8.cells()
+    cm = np.array([com["x"][-1], com["y"][-1], com["z"][-1]], np.float64)
+    exact = np.sqrt(((p32 - (m32[:, None] * p32).sum(0) / m32.sum()) ** 2).sum(1)).max()
+    assert abs(o8.cell_moments(cell)[7] - exact) <= 1e-12 * exact, (o8.cell_moments(cell)[7], exact, cm)
+
+
+def test_relative_test_inert_without_history_and_exact_in_the_limit(oracle):
+    from nbody_amd import universe as U
+
+    n = 4096
+    st = U.gaussian_clusters(n, seed=4)
+    o = _tree(oracle, st)
+    srt = o.sorted()
+    # step 0 after the tree stages: no previous acceleration, alpha changes nothing
+    a0, f0, _, n0 = o.coarse(0.9, alpha=0.0)
+    a1, f1, _, n1 = o.coarse(0.9, alpha=0.02)
+    assert np.array_equal(a0, a1) and np.array_equal(f0, f1) and n0 == n1
+    # with previous accelerations and alpha = 1e-18, a cell is taken only where
+    # its left-out term m b^3 / s^5 is below 1e-18 |a|: the FP64 direct sum
+    ref = oracle.direct_fp64(st[0], st[1], st[2], st[6], 0.01, np.arange(n))
+    o2 = oracle.Oracle(n, theta=0.5, softening=0.01, dt=1e-3, vote=64, flags=1)
+    o2.set_state(*st)
+    o2.set_accel(*(np.ascontiguousarray(ref[:, j], np.float32) for j in range(3)), 1)
+    assert o2.run_stages(1, 4) == 0
+    a2, _, _, n2 = o2.coarse(0.9, features=False, alpha=1e-18)
+    assert n2 > 0.99 * n * n
+    e = np.linalg.norm(a2[:n] - ref[srt], axis=1) / np.linalg.norm(ref[srt], axis=1)
+    assert e.max() < 1e-6, e.max()            # FP32 offsets d, as in the force stage
+    # alpha = 0.02 opens more than the extent test alone, for a smaller error
+    a3, _, _, n3 = o2.coarse(0.9, features=False, alpha=0.02)
+    a4, _, _, n4 = o2.coarse(0.9, features=False, alpha=0.0)
+    med = lambda a: np.median(np.linalg.norm(a[:n] - ref[srt], axis=1) / np.linalg.norm(ref[srt], axis=1))
+    assert n3 > n4 and med(a3) < med(a4)
 
 
 def test_features_mass_scaling_invariance_and_identities(oracle):
     from nbody_amd import universe as U
 
     st = list(U.plummer(4096, seed=5))
-    o1 = _tree(oracle, st)
-    a1, f1, e21, _ = o1.coarse(0.9)
+    n = len(st[0])
+
+    def run(state, scale):
+        o = oracle.Oracle(n, theta=0.5, softening=0.01, dt=1e-3, vote=64, flags=1)
+        o.set_state(*state)
+        # previous accelerations: the geometric traversal of a first tree
+        assert o.run_stages(1, 4) == 0
+        ap, _, _, _ = o.coarse(0.5, features=False, geometric=True)
+        srt = o.sorted()
+        prev = np.zeros((n, 3), np.float32)
+        prev[srt] = ap[:n]
+        o.set_accel(*(np.ascontiguousarray(prev[:, j]) for j in range(3)), 1)
+        assert o.run_stages(1, 4) == 0
+        return o.coarse(0.9, alpha=0.02)
+
+    a1, f1, e21, n1 = run(st, 1)
     st2 = st.copy()
     st2[6] = (st[6] * F(2)).astype(np.float32)
-    o2 = _tree(oracle, st2)
-    a2, f2, e22, _ = o2.coarse(0.9)
+    a2, f2, e22, n2 = run(st2, 2)
+    assert n1 == n2
     assert np.array_equal(a2, 2 * a1)
     assert np.array_equal(f1, f2)
     assert np.array_equal(e21, e22)
     # frame identities
     assert (f1[:, 4] >= 0).all()
     lhs = f1[:, 3].astype(np.float64) ** 2 + f1[:, 4].astype(np.float64) ** 2
-    rhs = np.exp(2 * f1[:, 15].astype(np.float64))
+    rhs = np.exp(2 * f1[:, 18].astype(np.float64))
     assert np.allclose(lhs, rhs, rtol=1e-4, atol=0)
     # e2 is a unit vector orthogonal to the coarse acceleration
     assert np.allclose(np.linalg.norm(e21, axis=1), 1.0, atol=1e-12)
     assert np.abs((e21 * a1).sum(1)).max() <= 1e-12 * np.linalg.norm(a1, axis=1).max()
+    # group sums that vanish: deviations from the group's mean acceleration and
+    # offsets from its centroid, taken back out of the frame
+    A = np.linalg.norm(a1, axis=1, keepdims=True)
+    e1 = a1 / A
+    e3 = np.cross(e1, e21)
+    back = lambda f: f[:, 0:1] * e1 + f[:, 1:2] * e21 + f[:, 2:3] * e3
+    dev = (back(f1[:, 5:8].astype(np.float64)) * A).reshape(-1, 64, 3).sum(1)
+    assert np.abs(dev).max() <= 1e-5 * A.max(), np.abs(dev).max()
+    off = back(f1[:, 8:11].astype(np.float64)).reshape(-1, 64, 3).sum(1)
+    assert np.abs(off).max() <= 1e-4, np.abs(off).max()
 
 
 # ---- network ---------------------------------------------------------------
