@@ -25,19 +25,17 @@
 // acceleration a: e1 = a / |a|, e2 = the part of the quadrupole acceleration
 // a_q perpendicular to e1, normalized, e3 = e1 x e2. Vectors are projected on
 // the frame and divided by |a|, so the features do not change under rotations
-// and under a common scaling of the masses. Only four sums per accepted cell;
-// the rest comes once per body from wave reductions.
+// and under a common scaling of the masses. Per accepted cell the traversal
+// only keeps the quadrupole part and the cell mass apart; the rest comes once
+// per body from wave reductions.
 //    0-2  acceleration from bodies (a_b)
 //    3-4  a_q along e1, e2 (its e3 component is 0 by construction)
 //    5-7  a minus the mean coarse acceleration of the group
 //    8-10 offset of the body from its group's centroid, over the group radius
 //   11-13 the group's mean a_b
-//   14-15 log(1 + sum u), log(1 + sum u^2) over accepted cells, with
-//         u = (b^2 / theta_c^2 + eps^2) / s^2 in (0, 1]: how close the cell
-//         came to failing the extent test
-//   16-17 log(1 + accepted cells), log(1 + bodies met)
-//   18-19 log(|a_q| / |a|), log(|a_b| / |a|)
-//   20    mass-weighted mean u of the accepted cells
+//   14-16 the group's mean a_q
+//   17-18 log(1 + accepted cells), log(1 + bodies met)
+//   19-20 log(|a_q| / |a|), log(|a_b| / |a|)
 //   21    accepted-cell mass / (accepted-cell mass + body mass met)
 //   22    log(group radius / root radius)
 //   23    cosine between the cell monopole part and a_b
@@ -237,7 +235,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f; // total, Kahan
     float bx = 0.f, by = 0.f, bz = 0.f;                                // bodies
     float qx = 0.f, qy = 0.f, qz = 0.f;                                // quadrupole part
-    float su = 0.f, sv = 0.f, smu = 0.f, smc = 0.f, smb = 0.f;
+    float smc = 0.f, smb = 0.f;
     int nc = 0, nbod = 0;
     long long ninter = 0;
     bool bad = false;
@@ -314,10 +312,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
                     pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
                     pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
-                    const float u = thr * r2i;
-                    su += u;
-                    sv = fmaf(u, u, sv);
-                    smu = fmaf(u, q[k].w, smu);
                     smc += q[k].w;
                     ++nc;
                 } else {
@@ -365,8 +359,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         float gx = valid ? me.x : 0.f, gy = valid ? me.y : 0.f, gz = valid ? me.z : 0.f;
         float gax = valid ? ax : 0.f, gay = valid ? ay : 0.f, gaz = valid ? az : 0.f;
         float gbx = valid ? bx : 0.f, gby = valid ? by : 0.f, gbz = valid ? bz : 0.f;
+        float gqx = valid ? qx : 0.f, gqy = valid ? qy : 0.f, gqz = valid ? qz : 0.f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
+            gqx += __shfl_xor(gqx, o);
+            gqy += __shfl_xor(gqy, o);
+            gqz += __shfl_xor(gqz, o);
             gx += __shfl_xor(gx, o);
             gy += __shfl_xor(gy, o);
             gz += __shfl_xor(gz, o);
@@ -386,6 +384,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         gbx /= nv;
         gby /= nv;
         gbz /= nv;
+        gqx /= nv;
+        gqy /= nv;
+        gqz /= nv;
         const float ox = me.x - gx, oy = me.y - gy, oz = me.z - gz;
         float rg = valid ? sqrtf(ox * ox + oy * oy + oz * oz) : 0.f;
 #pragma unroll
@@ -435,15 +436,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 f[10] = (ox * e3x + oy * e3y + oz * e3z) * ir;
             }
             NB_LOC(gbx, gby, gbz, 11);
-            f[14] = log1pf(su);
-            f[15] = log1pf(sv);
-            f[16] = log1pf((float)nc);
-            f[17] = log1pf((float)nbod);
+            NB_LOC(gqx, gqy, gqz, 14);
+            f[17] = log1pf((float)nc);
+            f[18] = log1pf((float)nbod);
             const float qn = sqrtf(qx * qx + qy * qy + qz * qz);
             const float bn = sqrtf(bx * bx + by * by + bz * bz);
-            f[18] = logf(fmaxf(qn * inv, 1e-30f));
-            f[19] = logf(fmaxf(bn * inv, 1e-30f));
-            f[20] = smc > 0.f ? smu / smc : 0.f;
+            f[19] = logf(fmaxf(qn * inv, 1e-30f));
+            f[20] = logf(fmaxf(bn * inv, 1e-30f));
             f[21] = (smc + smb) > 0.f ? smc / (smc + smb) : 0.f;
             f[22] = logf(fmaxf(rg / root_r, 1e-30f));
             const float mx = ax - bx - qx, my = ay - by - qy, mz = az - bz - qz;

@@ -931,8 +931,8 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
     const int cnt = (first + W <= n) ? W : n - first;
     if (cnt <= 0) return;
     float px[256], py[256], pz[256], aa[256];
-    /* per lane: total, bodies, quadrupole part; u sums */
-    double A[256][3], Bd[256][3], Qp[256][3], su[256], sv[256], smu[256];
+    /* per lane: total, bodies, quadrupole part */
+    double A[256][3], Bd[256][3], Qp[256][3];
     double smc = 0.0, smb = 0.0;
     long long nc = 0, nbod = 0;
     int stackNode[ORC_MAXDEPTH + 1], stackPos[ORC_MAXDEPTH + 1];
@@ -940,7 +940,6 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
         const int b = s->sorted[first + l];
         px[l] = X[b]; py[l] = Y[b]; pz[l] = Z[b];
         for (int j = 0; j < 3; ++j) A[l][j] = Bd[l][j] = Qp[l][j] = 0.0;
-        su[l] = sv[l] = smu[l] = 0.0;
         const float a2 = fmaf(s->ax[b], s->ax[b], fmaf(s->ay[b], s->ay[b], s->az[b] * s->az[b]));
         aa[l] = (cfg->alpha2 > 0.0f && s->step > 0 && a2 > 0.0f) ? cfg->alpha2 * a2 : INFINITY;
     }
@@ -1008,24 +1007,20 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
                 const double gx = c2 * dx - r5 * Qx, gy = c2 * dy - r5 * Qy, gz = c2 * dz - r5 * Qz;
                 A[l][0] += gx; A[l][1] += gy; A[l][2] += gz;
                 Qp[l][0] += gx; Qp[l][1] += gy; Qp[l][2] += gz;
-                const double u = (double)thr * r2i;
-                su[l] += u;
-                sv[l] += u * u;
-                smu[l] += u * mc;
             }
             ni += cnt;
         }
         depth--;
     }
     *inter += ni;
-    /* group centroid and radius, mean acceleration and mean body part */
-    double gx = 0, gy = 0, gz = 0, rg = 0, am[3] = {0, 0, 0}, bm[3] = {0, 0, 0};
+    /* group centroid and radius, mean acceleration, mean body and quadrupole parts */
+    double gx = 0, gy = 0, gz = 0, rg = 0, am[3] = {0, 0, 0}, bm[3] = {0, 0, 0}, qm[3] = {0, 0, 0};
     for (int l = 0; l < cnt; ++l) {
         gx += px[l]; gy += py[l]; gz += pz[l];
-        for (int j = 0; j < 3; ++j) { am[j] += A[l][j]; bm[j] += Bd[l][j]; }
+        for (int j = 0; j < 3; ++j) { am[j] += A[l][j]; bm[j] += Bd[l][j]; qm[j] += Qp[l][j]; }
     }
     gx /= cnt; gy /= cnt; gz /= cnt;
-    for (int j = 0; j < 3; ++j) { am[j] /= cnt; bm[j] /= cnt; }
+    for (int j = 0; j < 3; ++j) { am[j] /= cnt; bm[j] /= cnt; qm[j] /= cnt; }
     for (int l = 0; l < cnt; ++l) {
         const double ox = px[l] - gx, oy = py[l] - gy, oz = pz[l] - gz;
         const double r = sqrt(ox * ox + oy * oy + oz * oz);
@@ -1076,15 +1071,13 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
                 ORC_LOC(o3, 8, 1.0 / rg);
             }
             ORC_LOC(bm, 11, inv);
-            f[14] = (float)log1p(su[l]);
-            f[15] = (float)log1p(sv[l]);
-            f[16] = (float)log1p((double)nc);
-            f[17] = (float)log1p((double)nbod);
+            ORC_LOC(qm, 14, inv);
+            f[17] = (float)log1p((double)nc);
+            f[18] = (float)log1p((double)nbod);
             const double qn = sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2]);
             const double bn = sqrt(Bd[l][0] * Bd[l][0] + Bd[l][1] * Bd[l][1] + Bd[l][2] * Bd[l][2]);
-            f[18] = (float)log(fmax(qn * inv, 1e-30));
-            f[19] = (float)log(fmax(bn * inv, 1e-30));
-            f[20] = smc > 0.0 ? (float)(smu[l] / smc) : 0.0f;
+            f[19] = (float)log(fmax(qn * inv, 1e-30));
+            f[20] = (float)log(fmax(bn * inv, 1e-30));
             f[21] = (smc + smb) > 0.0 ? (float)(smc / (smc + smb)) : 0.0f;
             f[22] = (float)log(fmax(rg / (double)s->radius, 1e-30));
             const double mv[3] = {a[0] - Bd[l][0] - qv[0], a[1] - Bd[l][1] - qv[1], a[2] - Bd[l][2] - qv[2]};

@@ -18,9 +18,11 @@ counterpart, so the oracle is checked against what the mathematics fixes:
               alpha adds interactions and lowers the error
   features    invariance under scaling all masses (and the previous
               accelerations) by 2 (bit-exact); the identities
-              f3^2 + f4^2 = (|a_q| / |a|)^2 = exp(2 f18), f4 >= 0; and the group
+              f3^2 + f4^2 = (|a_q| / |a|)^2 = exp(2 f19), f4 >= 0; the group
               sums that vanish: sum |a| (f5 e1 + f6 e2 + f7 e3) = sum (a - mean a)
-              and sum (f8 e1 + f9 e2 + f10 e3) = sum (x - centroid) / r_g
+              and sum (f8 e1 + f9 e2 + f10 e3) = sum (x - centroid) / r_g; and
+              the group means taken back out of each body's frame, the same
+              vector on every lane and equal to the mean of the per-body parts
   network     wiring by one-path networks (a transposed or misindexed weight
               moves the output), GELU at known points, and the backward pass
               against central finite differences of the forward pass
@@ -212,7 +214,7 @@ def test_features_mass_scaling_invariance_and_identities(oracle):
     # frame identities
     assert (f1[:, 4] >= 0).all()
     lhs = f1[:, 3].astype(np.float64) ** 2 + f1[:, 4].astype(np.float64) ** 2
-    rhs = np.exp(2 * f1[:, 18].astype(np.float64))
+    rhs = np.exp(2 * f1[:, 19].astype(np.float64))
     assert np.allclose(lhs, rhs, rtol=1e-4, atol=0)
     # e2 is a unit vector orthogonal to the coarse acceleration
     assert np.allclose(np.linalg.norm(e21, axis=1), 1.0, atol=1e-12)
@@ -227,6 +229,13 @@ def test_features_mass_scaling_invariance_and_identities(oracle):
     assert np.abs(dev).max() <= 1e-5 * A.max(), np.abs(dev).max()
     off = back(f1[:, 8:11].astype(np.float64)).reshape(-1, 64, 3).sum(1)
     assert np.abs(off).max() <= 1e-4, np.abs(off).max()
+    # a_b from features 0-2, a_q from 3-4; the group means from 11-13 and 14-16
+    ab = (back(f1[:, 0:3].astype(np.float64)) * A).reshape(-1, 64, 3)
+    aq = ((f1[:, 3:4] * e1 + f1[:, 4:5] * e21) * A).reshape(-1, 64, 3)
+    for part, cols in ((ab, slice(11, 14)), (aq, slice(14, 17))):
+        gm = (back(f1[:, cols].astype(np.float64)) * A).reshape(-1, 64, 3)
+        scale = np.linalg.norm(part.mean(1), axis=1).max() + 1e-300
+        assert np.abs(gm - part.mean(1, keepdims=True)).max() <= 1e-5 * scale
 
 
 # ---- network ---------------------------------------------------------------
