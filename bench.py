@@ -13,14 +13,14 @@ must reach a median relative force error <= 1e-3 against the FP64 direct sum
 on 4096 sampled bodies:
   plain   Barnes-Hut at theta = 0.4 (the reference's opening test)
   learned coarse traversal (monopole + softened quadrupole; a cell is taken
-          when |d| >= b / theta_c for its extent b and its left-out term
-          m b^3 / s^5 is within alpha = 0.02 of the body's previous |a|) plus
-          the learned correction, trained on the GPU on 50 steps of
-          theta = 0.4 residuals before the timed region; the coarse angle
-          theta_c is the largest of 0.99, 0.97, 0.95, 0.925, 0.9, 0.85 whose
-          calibration error (a separate sample) is <= 0.9e-3. The sampled FP64
-          check (1024 bodies every 10 steps, tolerance 1e-3, plain theta as the
-          fallback) runs inside the timed steps.
+          when |d| >= b / theta_c for its extent b, theta_c = 0.99, and its
+          left-out term m b^3 / s^5 is within alpha of the body's previous
+          |a|) plus the learned correction, trained on the GPU on 50 steps of
+          theta = 0.4 residuals before the timed region; alpha is the largest
+          of 0.03, 0.025, 0.0225, 0.02, 0.0175, 0.015, 0.0125 whose calibration
+          error (a separate sample) is <= 0.9e-3. The sampled FP64 check (1024
+          bodies every 10 steps, tolerance 1e-3, plain theta as the fallback)
+          runs inside the timed steps.
 `value` is the steps/s of the requested mode (learned by default); the other
 mode's steps/s and both errors are in `config`.
 
@@ -40,8 +40,8 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-nbody_amd"))
 METRIC = "steps/s @4M bodies: learned coarse-θ vs plain θ=0.4 at equal 1e-3 force error"
 N_BODIES = 4_194_304
 THETA_PLAIN = 0.4
-THETA_COARSE_CANDIDATES = (0.99, 0.97, 0.95, 0.925, 0.9, 0.85)
-ALPHA_COARSE = 0.02
+THETA_COARSE = 0.99
+ALPHA_CANDIDATES = (0.03, 0.025, 0.0225, 0.02, 0.0175, 0.015, 0.0125)
 CALIBRATION_TARGET = 0.9e-3
 ERROR_BAR = 1e-3
 TRAIN = dict(steps=50, groups_per_step=512, iters=4000, batch=4096, seed=1)
@@ -58,34 +58,34 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(state, mode, theta_coarse=0.95):
+def make_sim(state, mode, alpha=0.02):
     from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
     from nbody_amd.universe import ArrayUniverse
 
     kw = dict(theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote_width=VOTE,
               flags=_lib.NB_FP64_POSITIONS | _lib.NB_FUSED_UPDATE)
     if mode == "learned":
-        kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=theta_coarse, coarse_alpha=ALPHA_COARSE, **CHECK)
+        kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE, coarse_alpha=alpha, **CHECK)
     sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
     sim.init(None)
     return sim
 
 
 def train_learned(state, candidates, train):
-    """Train one network per candidate coarse angle (on the GPU, on the
-    simulation's own pairs) and keep the coarsest whose error on a calibration
-    sample meets the target."""
+    """Train one network per candidate alpha (on the GPU, on the simulation's
+    own pairs) and keep the coarsest whose error on a calibration sample meets
+    the target."""
     chosen = None
-    for th in candidates:
-        sim = make_sim(state, "learned", th)
+    for al in candidates:
+        sim = make_sim(state, "learned", al)
         t0 = time.perf_counter()
         rep = sim.train(**train)
         rep["wall_s"] = time.perf_counter() - t0
         cal = sim.force_error(4096, seed=4242)
-        log(f"[learned] coarse theta {th}: calibration median {cal['median']:.3e}, train {rep}")
+        log(f"[learned] theta_c {THETA_COARSE} alpha {al}: calibration median {cal['median']:.3e}, train {rep}")
         if chosen is not None:
             chosen[0].close()
-        chosen = (sim, th, rep, cal)
+        chosen = (sim, al, rep, cal)
         if cal["median"] <= CALIBRATION_TARGET:
             break
     return chosen
@@ -154,8 +154,8 @@ def main():
     ap.add_argument("--mode", choices=["learned", "plain"], default=None)
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--theta-coarse", type=float, default=None,
-                    help="use this coarse angle instead of calibrating one")
+    ap.add_argument("--alpha", type=float, default=None,
+                    help="use this relative-test alpha instead of calibrating one")
     ap.add_argument("--train-iters", type=int, default=None, help="Adam iterations (default 4000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the vertex copy kernel once per mode (a copy of known size for HBM counters)")
@@ -182,9 +182,9 @@ def main():
             train = dict(TRAIN)
             if args.train_iters is not None:
                 train["iters"] = args.train_iters
-            cands = (args.theta_coarse,) if args.theta_coarse else THETA_COARSE_CANDIDATES
-            sim, th, rep, cal = train_learned(state, cands, train)
-            learned_info = dict(theta_coarse=th, train=rep, calibration_median=cal["median"])
+            cands = (args.alpha,) if args.alpha else ALPHA_CANDIDATES
+            sim, al, rep, cal = train_learned(state, cands, train)
+            learned_info = dict(alpha=al, train=rep, calibration_median=cal["median"])
         else:
             sim = make_sim(state, "plain")
             sim.steps(TRAIN["steps"])  # the same simulated time as the learned run's training
@@ -262,8 +262,8 @@ def main():
         p = results["plain"]
         tr = learned_info["train"]
         line["config"].update({
-            "theta_coarse": learned_info["theta_coarse"],
-            "alpha_coarse": ALPHA_COARSE,
+            "theta_coarse": THETA_COARSE,
+            "alpha_coarse": learned_info["alpha"],
             "plain_steps_per_s": round(p["steps_per_s"], 4),
             "plain_ms_per_step": round(p["ms_per_step"], 3),
             "speedup_vs_plain": round(main_r["steps_per_s"] / p["steps_per_s"], 3),
