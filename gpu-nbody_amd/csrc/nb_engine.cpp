@@ -32,7 +32,7 @@ struct nb_sim {
     std::vector<hipEvent_t> tev; // per-step stage events (timing mode)
     double stage_ms[8] = {};
     unsigned long long inter_before = 0;
-    long long last_interactions = 0;
+    long long last_interactions = 0, last_cell_interactions = 0, cell_inter_before = 0;
     long long fallbacks = 0;
     double last_check_median = 0.0;
     int *chk_idx = nullptr;             // sampled-error check buffers, allocated on first use
@@ -130,6 +130,7 @@ nb::Ctl initial_ctl(const nb_sim *s, uint32_t epoch)
     c.error = 0;
     c.bbox_count = 0;
     c.interactions = 0;
+    c.cell_interactions = 0;
     (void)s;
     return c;
 }
@@ -148,6 +149,8 @@ int check_device_error(nb_sim *s)
     if (r) return r;
     s->last_interactions = (long long)(c.interactions - s->inter_before);
     s->inter_before = c.interactions;
+    s->last_cell_interactions = (long long)(c.cell_interactions - s->cell_inter_before);
+    s->cell_inter_before = c.cell_interactions;
     switch (c.error) {
     case nb::kErrNone: return NB_OK;
     case nb::kErrPool: return fail(s, NB_E_POOL, "octree cell pool exhausted (%d cells)", s->d.cap);
@@ -438,7 +441,7 @@ int checked_force(nb_sim *s)
     std::vector<int32_t> idx((size_t)ns);
     for (int i = 0; i < ns; ++i) idx[i] = (int32_t)(splitmix64(s->check_seed) % (uint64_t)n);
     HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, s->stream);
+    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, false, s->stream);
     if (e == hipSuccess) e = nb::launch_gather_acc(s->d, s->chk_idx, ns, s->chk_acc, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "sampled check launch: %s", hipGetErrorString(e));
     std::vector<double4> ref;
@@ -612,6 +615,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     HIP_TRY(s, hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     s->inter_before = 0;
+    s->cell_inter_before = 0;
     s->have_state = true;
     s->err.clear();
     return NB_OK;
@@ -858,7 +862,7 @@ static int direct_accel(nb_sim *s, const int32_t *idx, int32_t ns, std::vector<d
     int r = ensure_check_buffers(s, ns);
     if (r) return r;
     HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx, (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, s->stream);
+    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, false, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "direct-sum launch: %s", hipGetErrorString(e));
     return fetch(s, out, (const double4 *)s->chk_out, (size_t)ns);
 }
@@ -1293,7 +1297,7 @@ int nb_diagnostics(nb_sim *s, int32_t pe_samples, uint64_t seed, double *out)
         std::vector<double4> res;
         HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data() + b0, (size_t)nb_ * sizeof(int), hipMemcpyHostToDevice,
                                   s->stream));
-        e = nb::launch_direct64(s->d, s->chk_idx, nb_, s->chk_part, s->chk_out, s->stream);
+        e = nb::launch_direct64(s->d, s->chk_idx, nb_, s->chk_part, s->chk_out, true, s->stream);
         if (e != hipSuccess) return fail(s, NB_E_HIP, "direct-sum launch: %s", hipGetErrorString(e));
         r = fetch(s, res, (const double4 *)s->chk_out, (size_t)nb_);
         if (r) return r;
@@ -1375,6 +1379,7 @@ int nb_get_counters(nb_sim *s, nb_counters *out)
     out->max_depth = c.max_depth;
     out->live_cells = c.live_cells;
     out->interactions = s->last_interactions;
+    out->cell_interactions = s->last_cell_interactions;
     out->error = c.error;
     out->fallbacks = s->fallbacks;
     for (int i = 0; i < 8; ++i) out->stage_ms[i] = s->stage_ms[i];

@@ -221,20 +221,24 @@ hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
 
 // ---------------------------------------------------------------------------
 // FP64 direct sum on sampled bodies (the error yardstick; G = 1), with the
-// softened potential sum_{j != i} m_j / sqrt(r^2 + eps^2) in the w component.
-// Block (t, s) sums the bodies [s * chunk, (s + 1) * chunk) for 64 sampled
-// targets: 4 threads per target stride through the chunk, whose bodies are
-// staged through LDS once for all 64 targets. Partial sums per (target,
-// chunk) are combined by a second kernel in a fixed order, so the result is
-// deterministic.
+// softened potential sum_{j != i} m_j / sqrt(r^2 + eps^2) in the w component
+// when PHI (else w = 0). Block (t, s) sums the bodies [s * chunk, (s + 1) *
+// chunk) for 64 sampled targets: 4 threads per target stride through the
+// chunk, whose bodies are staged through LDS (converted to FP64) once for all
+// 64 targets. 1 / sqrt(r^2) starts from the FP32 hardware estimate (relative
+// error < 2^-22 on the FP32-rounded argument) and takes one FP64 Newton step,
+// y (3/2 - r^2 y^2 / 2), which leaves a relative error below 1e-13. Partial
+// sums per (target, chunk) are combined by a second kernel in a fixed order,
+// so the result is deterministic.
 // ---------------------------------------------------------------------------
 constexpr int kDirChunks = 64;  // source chunks
 constexpr int kDirTargets = 64; // targets per block
 constexpr int kDirTile = 256;   // bodies staged per LDS tile
 
+template <bool PHI>
 __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__ idx, int ns, double4 *part)
 {
-    __shared__ float4 tile[kDirTile];
+    __shared__ double4 tile[kDirTile];
     __shared__ double4 red[256];
     const int n = d.n;
     const int t0 = blockIdx.x * kDirTargets;
@@ -258,19 +262,21 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
     for (int base = lo; base < hi; base += kDirTile) {
         __syncthreads();
         const int j = base + threadIdx.x;
-        tile[threadIdx.x] = j < hi ? d.pm4[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 pj = j < hi ? d.pm4[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        tile[threadIdx.x] = make_double4(pj.x, pj.y, pj.z, pj.w);
         __syncthreads();
         const int cnt = min(kDirTile, hi - base);
         for (int k = sub; k < cnt; k += 4) {
-            const float4 s = tile[k];
-            const double dx = (double)s.x - tx, dy = (double)s.y - ty, dz = (double)s.z - tz;
+            const double4 s = tile[k];
+            const double dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
             const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-            const double ri = rsqrt(r2);
-            const double f = (double)s.w * ri * ri * ri;
+            double ri = (double)__builtin_amdgcn_rsqf((float)r2);
+            ri = ri * fma(-0.5 * r2, ri * ri, 1.5);
+            const double f = s.w * ri * ri * ri;
             ax = fma(dx, f, ax);
             ay = fma(dy, f, ay);
             az = fma(dz, f, az);
-            phi = fma((double)s.w, base + k != self ? ri : 0.0, phi);
+            if (PHI) phi = fma(s.w, base + k != self ? ri : 0.0, phi);
         }
     }
     red[threadIdx.x] = make_double4(ax, ay, az, phi);
@@ -298,11 +304,15 @@ __global__ __launch_bounds__(256) void k_direct64_reduce(const double4 *part, in
 }
 
 // part must hold ns * 64 entries, out ns entries
-hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, hipStream_t st)
+hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, bool phi,
+                           hipStream_t st)
 {
     if (ns <= 0) return hipSuccess;
     const dim3 grid((ns + kDirTargets - 1) / kDirTargets, kDirChunks);
-    hipLaunchKernelGGL(k_direct64, grid, dim3(256), 0, st, d, idx, ns, part);
+    if (phi)
+        hipLaunchKernelGGL(k_direct64<true>, grid, dim3(256), 0, st, d, idx, ns, part);
+    else
+        hipLaunchKernelGGL(k_direct64<false>, grid, dim3(256), 0, st, d, idx, ns, part);
     hipLaunchKernelGGL(k_direct64_reduce, dim3((ns + 255) / 256), dim3(256), 0, st, (const double4 *)part, ns,
                        out);
     return hipGetLastError();

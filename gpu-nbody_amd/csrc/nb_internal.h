@@ -70,7 +70,7 @@ struct Ctl {
     unsigned long long interactions; // cumulative body-node interactions
     int levels;          // number of non-empty cell levels of the last tree
     int live_cells;      // cells linked into the tree by the last build
-    int pad[2];
+    unsigned long long cell_interactions; // cumulative cell interactions of the coarse traversal
     int lvl_start[kMaxDepth + 2]; // first lvl_list entry of each level
 };
 
@@ -114,7 +114,9 @@ hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);
 // FP64 direct-sum accelerations of the bodies idx[0..ns) (G = 1), into out;
 // out[i].w = sum_{j != idx[i]} m_j / sqrt(r^2 + eps^2) (the softened potential, negated)
-hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, hipStream_t st);
+// when phi, else 0
+hipError_t launch_direct64(const Dev &d, const int *idx, int ns, double4 *part, double4 *out, bool phi,
+                           hipStream_t st);
 hipError_t launch_integrate(const Dev &d, hipStream_t st);
 // FP64 per-block sums of m, m v, m v^2 / 2, m x cross v, m x (11 doubles per block)
 constexpr int kMomentBlocks = 512;

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     float qx = 0.f, qy = 0.f, qz = 0.f;                                // quadrupole part
     float smc = 0.f, smb = 0.f;
     int nc = 0, nbod = 0;
-    long long ninter = 0;
+    long long ninter = 0, ncell = 0;
     bool bad = false;
     while (sp > 0) {
         __builtin_amdgcn_wave_barrier();
@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 const float r5 = r3 * r2i;
                 const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
                 ++cnt;
+                ++ncell;
                 if (kFeatOn) {
                     pmx = fmaf(dx, f, pmx);
                     pmy = fmaf(dy, f, pmy);
@@ -494,9 +495,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     }
     // device interaction counter: one atomic per wave
     const unsigned long long nvalid = (unsigned long long)__popcll(__ballot(valid));
-    if (lane == 0 && ninter && nvalid)
+    if (lane == 0 && ninter && nvalid) {
         __hip_atomic_fetch_add(&ctl->interactions, (unsigned long long)ninter * nvalid, __ATOMIC_RELAXED,
                                AGENT);
+        __hip_atomic_fetch_add(&ctl->cell_interactions, (unsigned long long)ncell * nvalid, __ATOMIC_RELAXED,
+                               AGENT);
+    }
 }
 
 hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,

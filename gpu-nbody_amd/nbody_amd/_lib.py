@@ -69,7 +69,8 @@ class NbCounters(C.Structure):
         ("stage_ms", C.c_double * 8),
         ("last_check_median", C.c_double),
         ("live_cells", C.c_int64),
-        ("reserved", C.c_int64 * 5),
+        ("cell_interactions", C.c_int64),
+        ("reserved", C.c_int64 * 4),
     ]
 
 

@@ -161,7 +161,9 @@ typedef struct nb_counters {
     double stage_ms[8];    /* per-step device time of each stage over the last nb_step call (timing on) */
     double last_check_median; /* learned mode: last sampled median relative error */
     int64_t live_cells;    /* cells linked into the tree by the last build */
-    int64_t reserved[5];
+    int64_t cell_interactions; /* learned and coarse modes: the part of `interactions` that
+                                  were cells (monopole + quadrupole) */
+    int64_t reserved[4];
 } nb_counters;
 int nb_get_counters(nb_sim *sim, nb_counters *out);
 
