@@ -66,11 +66,25 @@ def test_create_rejects_bad_params_without_gpu(nb, field, value):
     assert lib.nb_last_error(None) == b"invalid nb_params"
 
 
-def test_learned_mode_needs_coarser_theta(nb):
+@pytest.mark.parametrize("kw", [
+    dict(coarse_theta=0.3),                   # below theta
+    dict(coarse_theta=1.0),                   # the extent test needs theta_c < 1
+    dict(coarse_alpha=-0.01),                 # negative relative bound
+    dict(coarse_alpha=float("inf")),
+    dict(vote_width=16),                      # the coarse traversal is one wave64 per group
+])
+def test_learned_mode_parameter_checks(nb, kw):
     lib = nb.load()
-    p = nb.make_params(1000, theta=0.4, coarse_theta=0.3, method=nb._lib.NB_MODE_LEARNED)
+    args = dict(theta=0.4, coarse_theta=0.95, coarse_alpha=0.02, vote_width=64, method=nb._lib.NB_MODE_LEARNED)
+    args.update(kw)
+    p = nb.make_params(1000, **args)
     h = C.c_void_p()
     assert lib.nb_create(C.byref(p), C.byref(h)) == nb._lib.NB_E_ARG
+
+
+def test_default_learned_parameters(nb):
+    p = nb.make_params(1000)
+    assert np.isclose(p.coarse_theta, 0.95) and np.isclose(p.coarse_alpha, 0.02)
 
 
 def test_null_handles(nb):
