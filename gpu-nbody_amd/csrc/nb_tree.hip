@@ -448,17 +448,40 @@ __global__ __launch_bounds__(256) void k_level_scatter(Dev d)
 // reduction produces identically on every lane. Children are compacted to
 // the front (summarizetree.cl:75-81) into the traversal list.
 // ---------------------------------------------------------------------------
+// Lane exchanges inside each group of 8 lanes as DPP moves (no LDS):
+// quad_perm [1,0,3,2] = lane ^ 1, quad_perm [2,3,0,1] = lane ^ 2, and
+// row_half_mirror (lane i <-> 7 - i) to join the two halves of 8. After the
+// first two steps lanes 0-3 all hold (s0 + s1) + (s2 + s3) and lanes 4-7
+// (s4 + s5) + (s6 + s7) (floating-point addition commutes exactly), so the
+// third gives every lane the same pairwise sum as xor 4 would.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141;
+
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)(unsigned)u), hi = dpp_i<CTRL>((int)(unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 __device__ __forceinline__ double sum8(double v)
 {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
+    v += dpp_d<kDppXor1>(v);
+    v += dpp_d<kDppXor2>(v);
+    v += dpp_d<kDppHalfMirror>(v);
     return v;
 }
 
 template <bool QUAD>
 __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int cells)
 {
+    // products are rounded before they are summed, as in the oracle (no FMA
+    // contraction: it would also fold a product into the first exchange step)
+#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t >> 3, k = t & 7;
     const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
@@ -505,9 +528,9 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     const int at = v >= 0 ? below : used + (k - below);
     const double sm = sum8(m), sx = sum8(m * x), sy = sum8(m * y), sz = sum8(m * z);
     int scnt = cnt;
-    scnt += __shfl_xor(scnt, 1);
-    scnt += __shfl_xor(scnt, 2);
-    scnt += __shfl_xor(scnt, 4);
+    scnt += dpp_i<kDppXor1>(scnt);
+    scnt += dpp_i<kDppXor2>(scnt);
+    scnt += dpp_i<kDppHalfMirror>(scnt);
     // a cell of zero mass gets its centre at the origin instead of the
     // reference's 0 * inf = NaN (summarizetree.cl:161-167)
     const double inv = sm > 0.0 ? 1.0 / sm : 0.0;
@@ -538,9 +561,9 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
         // follow from these values are bit-exact against the oracle.
         const double dx = x - com.x, dy = y - com.y, dz = z - com.z;
         double br = v >= 0 ? sqrt(fma(dx, dx, fma(dy, dy, dz * dz))) + bc : 0.0;
-        br = fmax(br, __shfl_xor(br, 1));
-        br = fmax(br, __shfl_xor(br, 2));
-        br = fmax(br, __shfl_xor(br, 4));
+        br = fmax(br, dpp_d<kDppXor1>(br));
+        br = fmax(br, dpp_d<kDppXor2>(br));
+        br = fmax(br, dpp_d<kDppHalfMirror>(br));
         const double r2 = dx * dx + dy * dy + dz * dz;
         double t7[7];
         t7[0] = q[0] + m * (3.0 * dx * dx - r2);
