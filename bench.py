@@ -20,9 +20,11 @@ on 4096 sampled bodies:
           of 0.03, 0.025, 0.0225, 0.02, 0.0175, 0.015, 0.0125 whose calibration
           error (a separate sample) is <= 0.9e-3. The sampled FP64 check (1024
           bodies every 10 steps, tolerance 1e-3, plain theta as the fallback)
-          runs inside the timed steps.
+          runs inside the timed steps and counts in `value`; `config` also gives
+          its device time and the steps/s without it.
 `value` is the steps/s of the requested mode (learned by default); the other
-mode's steps/s and both errors are in `config`.
+mode's steps/s, both errors and each stage's fraction of the HBM roof are in
+`config`.
 
 Prints one JSON line. One GPU only: --gpus N > 1 exits with a message.
 """
@@ -62,12 +64,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(state, mode, alpha=0.02):
+def make_sim(state, mode, alpha=0.02, fused=True):
     from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
     from nbody_amd.universe import ArrayUniverse
 
     kw = dict(theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote_width=VOTE,
-              flags=_lib.NB_FP64_POSITIONS | _lib.NB_FUSED_UPDATE)
+              flags=_lib.NB_FP64_POSITIONS | (_lib.NB_FUSED_UPDATE if fused else 0))
     if mode == "learned":
         kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE, coarse_alpha=alpha, **CHECK)
     sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
@@ -97,12 +99,39 @@ def train_learned(state, candidates, train):
 
 def timed_run(sim, steps, warmup):
     """W untimed steps, then exactly K steps between two synchronizations
-    (nb_step returns after its stream has drained)."""
+    (nb_step returns after its stream has drained). Also returns the device
+    time and the number of the sampled FP64 checks inside the K steps."""
     sim.steps(warmup)
+    c0 = sim.counters()
     t0 = time.perf_counter()
     sim.steps(steps)
     t1 = time.perf_counter()
-    return steps / (t1 - t0), (t1 - t0) * 1e3 / steps
+    c1 = sim.counters()
+    return (steps / (t1 - t0), (t1 - t0) * 1e3 / steps, c1.check_ms - c0.check_ms,
+            int(c1.checks - c0.checks))
+
+
+def stage_rooflines(stages, n, cells, fused, learned):
+    """Per-stage fraction of the HBM roof from the algorithmic bytes of each
+    stage in the reference's FP32 layout (4-byte words; C = live cells): bbox
+    12N, build 16N + 52C, summarize 16N + 68C, sort 4N + 44C, integrate 96N
+    with FP64 positions. The stage times are the HIP-event times of the whole
+    stage (every launch of it and the gaps between them)."""
+    algo = {"bbox": 12 * n, "build": 16 * n + 52 * cells, "summarize": 16 * n + 68 * cells,
+            "sort": 4 * n + 44 * cells, "integrate": 96 * n}
+    out = {}
+    for k, b in algo.items():
+        ms = stages[k]
+        if k == "integrate" and fused:
+            out[k] = {"algorithmic_bytes": b, "note": "fused into the force kernel's epilogue"}
+            continue
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"algorithmic_bytes": b, "ms": ms, "GB/s": round(gbs, 1) if gbs else None,
+                  "frac": round(gbs / PEAK_HBM_GBS, 4) if gbs else None}
+    if learned:
+        out["summarize"]["note"] = ("counts the reference's monopole bytes; the learned mode also "
+                                    "sums second moments and extents (FP64)")
+    return out
 
 
 def stage_profile(sim, steps):
@@ -113,7 +142,7 @@ def stage_profile(sim, steps):
     c = sim.counters()
     names = ("bbox", "build", "summarize", "sort", "force", "integrate")
     return ({k: round(c.stage_ms[i], 4) for i, k in enumerate(names)}, c.interactions / steps,
-            c.cell_interactions / steps)
+            c.cell_interactions / steps, int(c.live_cells))
 
 
 def cpu_baseline(state, threads):
@@ -162,6 +191,8 @@ def main():
     ap.add_argument("--alpha", type=float, default=None,
                     help="use this relative-test alpha instead of calibrating one")
     ap.add_argument("--train-iters", type=int, default=None, help="Adam iterations (default 4000)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="plain mode with the integrate stage as its own kernel (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the vertex copy kernel once per mode (a copy of known size for HBM counters)")
     args = ap.parse_args()
@@ -191,15 +222,17 @@ def main():
             sim, al, rep, cal = train_learned(state, cands, train)
             learned_info = dict(alpha=al, train=rep, calibration_median=cal["median"])
         else:
-            sim = make_sim(state, "plain")
+            sim = make_sim(state, "plain", fused=not args.unfused)
             sim.steps(TRAIN["steps"])  # the same simulated time as the learned run's training
         log(f"[{m}] timing {args.steps} steps after {args.warmup} warm-up steps")
-        sps, ms = timed_run(sim, args.steps, args.warmup)
+        sps, ms, check_ms, checks = timed_run(sim, args.steps, args.warmup)
         fallbacks = sim.counters().fallbacks
-        stages, inter, cell_inter = stage_profile(sim, max(3, min(args.steps, 10)))
+        stages, inter, cell_inter, live = stage_profile(sim, max(3, min(args.steps, 10)))
         err = sim.force_error(args.samples, seed=1000)
         results[m] = dict(steps_per_s=sps, ms_per_step=ms, stages=stages, interactions=inter,
-                          cell_interactions=cell_inter, err=err, fallbacks=fallbacks)
+                          cell_interactions=cell_inter, err=err, fallbacks=fallbacks, check_ms=check_ms,
+                          checks=checks, live_cells=live,
+                          rooflines=stage_rooflines(stages, args.n, live, not args.unfused, m == "learned"))
         if args.copy_calibration:
             # k_copy_vertices reads 2 x 16 B and writes 2 x 16 B per body
             pos4 = np.zeros((args.n, 4), np.float32)
@@ -246,6 +279,8 @@ def main():
             "median_rel_force_err": round(main_r["err"]["median"], 8),
             "p90_rel_force_err": round(main_r["err"]["p90"], 8),
             "stage_ms": main_r["stages"],
+            "live_cells": main_r["live_cells"],
+            "stage_roofline": main_r["rooflines"],
             "interactions_per_step": int(main_r["interactions"]),
             "cell_interactions_per_step": int(main_r["cell_interactions"]),
             "interactions_per_s": round(main_r["interactions"] / (force_ms * 1e-3), 1),
@@ -280,7 +315,15 @@ def main():
             "plain_stage_ms": p["stages"],
             "both_within_1e-3": bool(main_r["err"]["median"] <= ERROR_BAR and p["err"]["median"] <= ERROR_BAR),
             "learned_calibration_median": round(learned_info["calibration_median"], 8),
-            "check": dict(CHECK, fallbacks_in_timed_steps=int(main_r["fallbacks"])),
+            "check": dict(CHECK, fallbacks_in_timed_steps=int(main_r["fallbacks"]),
+                          checks_in_timed_steps=main_r["checks"],
+                          check_ms_in_timed_steps=round(main_r["check_ms"], 3)),
+            # the sampled check's device time taken out of the timed steps
+            "steps_per_s_excluding_check": round(
+                args.steps / (args.steps * main_r["ms_per_step"] - main_r["check_ms"]) * 1e3, 4),
+            "speedup_vs_plain_excluding_check": round(
+                args.steps / (args.steps * main_r["ms_per_step"] - main_r["check_ms"]) * 1e3
+                / p["steps_per_s"], 3),
             "training": {"steps": TRAIN["steps"], "pairs": tr["samples"], "iters": TRAIN["iters"],
                          "batch": TRAIN["batch"], "pair_generation_ms": round(tr["gen_ms"], 1),
                          "adam_ms": round(tr["train_ms"], 1), "loss_first": round(tr["loss_first"], 4),

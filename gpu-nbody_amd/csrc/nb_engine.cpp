@@ -38,8 +38,13 @@ struct nb_sim {
     int *chk_idx = nullptr;             // sampled-error check buffers, allocated on first use
     double4 *chk_part = nullptr, *chk_out = nullptr;
     int chk_cap = 0;
-    int levels = 0;                      // non-empty cell levels of the current tree
-    int lvl_start[nb::kMaxDepth + 2] = {}; // host copy of the level list offsets
+    // level sizes of an earlier tree (the last one read back): they size the
+    // grids of the summarize and sort launches, which read the current sizes
+    // on the device, so no step waits for them
+    bool hint_valid = false;
+    int hint_levels = 0;
+    int hint_start[nb::kMaxDepth + 2] = {};
+    int split = 0; // levels split.. go to the single-block tail launches
     // learned mode
     nb::Learn L{};
     void *mlpbuf = nullptr;      // packed inference network (nb::kMlpBufBytes)
@@ -49,6 +54,9 @@ struct nb_sim {
     long long steps_since_model = 0;
     uint64_t check_seed = 0x5eed;
     float4 *chk_acc = nullptr;
+    hipEvent_t chk_ev[2] = {nullptr, nullptr}; // bracket the sampled check on the stream
+    double check_ms = 0.0;                     // device time of the sampled checks so far
+    long long checks = 0;
     std::string err;
 };
 
@@ -146,11 +154,21 @@ int read_ctl(nb_sim *s, nb::Ctl *out)
     return NB_OK;
 }
 
+// Keeps the level sizes of a tree read back to size later launches.
+void update_hint(nb_sim *s, const nb::Ctl &c)
+{
+    if (c.error || c.levels <= 0) return;
+    s->hint_valid = true;
+    s->hint_levels = c.levels;
+    std::memcpy(s->hint_start, c.lvl_start, sizeof s->hint_start);
+}
+
 int check_device_error(nb_sim *s)
 {
     nb::Ctl c;
     int r = read_ctl(s, &c);
     if (r) return r;
+    update_hint(s, c);
     s->last_interactions = (long long)(c.interactions - s->inter_before);
     s->inter_before = c.interactions;
     s->last_cell_interactions = (long long)(c.cell_interactions - s->cell_inter_before);
@@ -165,17 +183,31 @@ int check_device_error(nb_sim *s)
     }
 }
 
-// Groups the live cells by level and reads the level sizes back to the host
-// (the only host synchronization inside a step).
+int hint_cells(const nb_sim *s, int level)
+{
+    return level < s->hint_levels ? s->hint_start[level + 1] - s->hint_start[level] : 0;
+}
+
+// Levels whose expected size is at most this many cells go to the tail
+// launch from the bottom up (two passes of its block each).
+constexpr int kTailCells = 2 * nb::kTailThreads / 8;
+
+// Groups the live cells by level. Without level sizes of an earlier tree (the
+// first step of a state) it reads this tree's back, the only host wait inside
+// a step; later steps size their launches from the last sizes read.
 int prepare_levels(nb_sim *s)
 {
     hipError_t e = nb::launch_levels(s->d, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "level grouping launch: %s", hipGetErrorString(e));
-    nb::Ctl c;
-    int r = read_ctl(s, &c);
-    if (r) return r;
-    s->levels = c.error ? 0 : c.levels;
-    std::memcpy(s->lvl_start, c.lvl_start, sizeof s->lvl_start);
+    if (!s->hint_valid) {
+        nb::Ctl c;
+        int r = read_ctl(s, &c);
+        if (r) return r;
+        update_hint(s, c);
+    }
+    int split = s->hint_levels;
+    while (split > 0 && hint_cells(s, split - 1) <= kTailCells) --split;
+    s->split = split;
     return NB_OK;
 }
 
@@ -219,20 +251,19 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD:
-        e = nb::launch_build(d, s->build_blocks, s->stream);
-        s->levels = 0;
-        break;
+    case NB_STAGE_BUILD: e = nb::launch_build(d, s->build_blocks, s->stream); break;
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
-        for (int L = s->levels - 1; L >= 0 && e == hipSuccess; --L)
-            e = nb::launch_summarize_level(d, L, s->lvl_start[L + 1] - s->lvl_start[L], s->stream);
+        e = nb::launch_summarize_tail(d, s->split, s->stream);
+        for (int L = s->split - 1; L >= 0 && e == hipSuccess; --L)
+            e = nb::launch_summarize_level(d, L, hint_cells(s, L), s->stream);
         break;
     }
     case NB_STAGE_SORT:
-        for (int L = 0; L < s->levels && e == hipSuccess; ++L)
-            e = nb::launch_sort_level(d, L, s->lvl_start[L + 1] - s->lvl_start[L], s->stream);
+        for (int L = 0; L < s->split && e == hipSuccess; ++L)
+            e = nb::launch_sort_level(d, L, hint_cells(s, L), s->stream);
+        if (e == hipSuccess) e = nb::launch_sort_tail(d, s->split, s->stream);
         break;
     case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:
@@ -344,13 +375,15 @@ void free_all(nb_sim *s)
 {
     nb::Dev &d = s->d;
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
-                    d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list,
+                    d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list, d.clvl,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
                     s->mlpbuf, s->chk_acc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : s->chk_ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
@@ -442,17 +475,34 @@ int checked_force(nb_sim *s)
     const int ns = std::min(s->p.check_samples, n);
     r = ensure_check_buffers(s, ns);
     if (r) return r;
+    for (auto &ev : s->chk_ev)
+        if (!ev) HIP_TRY(s, hipEventCreate(&ev));
     std::vector<int32_t> idx((size_t)ns);
     for (int i = 0; i < ns; ++i) idx[i] = (int32_t)(splitmix64(s->check_seed) % (uint64_t)n);
     HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipEventRecord(s->chk_ev[0], s->stream));
     hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, false, s->stream);
     if (e == hipSuccess) e = nb::launch_gather_acc(s->d, s->chk_idx, ns, s->chk_acc, s->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->chk_ev[1], s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "sampled check launch: %s", hipGetErrorString(e));
     std::vector<double4> ref;
     std::vector<float4> got;
     r = fetch(s, ref, (const double4 *)s->chk_out, (size_t)ns);
     if (!r) r = fetch(s, got, (const float4 *)s->chk_acc, (size_t)ns);
     if (r) return r;
+    float ms = 0.f; // both events have completed: the fetches drained the stream
+    HIP_TRY(s, hipEventElapsedTime(&ms, s->chk_ev[0], s->chk_ev[1]));
+    s->check_ms += ms;
+    ++s->checks;
+    {
+        // a device error earlier in the step left the accelerations unset:
+        // report it instead of judging them
+        nb::Ctl c;
+        r = read_ctl(s, &c);
+        if (r) return r;
+        update_hint(s, c);
+        if (c.error) return check_device_error(s);
+    }
     std::vector<double> err((size_t)ns);
     for (int i = 0; i < ns; ++i) {
         const double dx = got[i].x - ref[i].x, dy = got[i].y - ref[i].y, dz = got[i].z - ref[i].z;
@@ -546,7 +596,7 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = ok && dalloc(&d.count, cap) == hipSuccess && dalloc(&d.start, cap) == hipSuccess;
     ok = ok && dalloc(&d.com64, cap) == hipSuccess;
     ok = ok && dalloc(&d.cchild, 8 * cap) == hipSuccess;
-    ok = ok && dalloc(&d.lvl_list, cap) == hipSuccess;
+    ok = ok && dalloc(&d.lvl_list, cap) == hipSuccess && dalloc(&d.clvl, cap) == hipSuccess;
     ok = ok && dalloc(&d.lvl_blockhist, (size_t)nb::kLevelBlocks * nb::kMaxDepth) == hipSuccess;
     ok = ok && dalloc(&d.cdata, 8 * cap) == hipSuccess;
     ok = ok && dalloc(&d.bnode, (size_t)n) == hipSuccess && dalloc(&d.probe, (size_t)n) == hipSuccess;
@@ -620,6 +670,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     s->inter_before = 0;
     s->cell_inter_before = 0;
+    s->hint_valid = false; // the first step reads its level sizes back
     s->have_state = true;
     s->err.clear();
     return NB_OK;
@@ -1367,6 +1418,7 @@ int nb_restore_state(nb_sim *s, const double *x, const double *y, const double *
     HIP_TRY(s, hipMemcpyAsync(&d.ctl->step, &step, sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, hipMemcpyAsync(&d.ctl->max_depth, &max_depth, sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->hint_valid = false;
     return NB_OK;
 }
 
@@ -1388,6 +1440,8 @@ int nb_get_counters(nb_sim *s, nb_counters *out)
     out->fallbacks = s->fallbacks;
     for (int i = 0; i < 8; ++i) out->stage_ms[i] = s->stage_ms[i];
     out->last_check_median = s->last_check_median;
+    out->check_ms = s->check_ms;
+    out->checks = s->checks;
     return NB_OK;
 }
 

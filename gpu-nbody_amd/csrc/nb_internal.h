@@ -45,6 +45,8 @@ constexpr int kMaxDepth = 64;        // reference MAXDEPTH, calculateforce.cl:12
 constexpr int kBuildLevelLimit = 64; // cells live on levels < 64, bodies at depth <= 64
 constexpr int kBBoxBlocks = 1024;    // partial-result slots of the bounding box
 constexpr int kLevelBlocks = 512;    // blocks of the level histogram / scatter
+constexpr int kTailThreads = 512;    // the single block of the deepest summarize / sort levels
+constexpr unsigned char kNotLive = 0xff; // clvl of a cell that is not in the tree
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 enum DevError : int {
@@ -96,6 +98,7 @@ struct Dev {
     float4 *cqb;
     float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^2) of the child's extent b
     int *lvl_list;
+    unsigned char *clvl; // cap: each cell's level, or kNotLive (level histogram -> scatter)
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
     float4 *bnode;
     int *sorted;
@@ -107,8 +110,15 @@ struct Dev {
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
 hipError_t launch_build(const Dev &d, int blocks, hipStream_t st);
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
-hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st);
-hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st);
+// Summarize and sort read each level's cell count on the device; the host's
+// cells_hint (the level's size in an earlier tree) only sizes the grid. The
+// tail launches take, in one block, every level from `split` to the tree's
+// last: summarize runs its tail before the levels split-1 .. 0, sort after
+// the levels 0 .. split-1.
+hipError_t launch_summarize_level(const Dev &d, int level, int cells_hint, hipStream_t st);
+hipError_t launch_summarize_tail(const Dev &d, int split, hipStream_t st);
+hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st);
+hipError_t launch_sort_tail(const Dev &d, int split, hipStream_t st);
 // mode: 0 = store accelerations (reference semantics), 1 = also integrate
 // (fused kick-drift-kick), 2 = probe: write accelerations to d.probe only
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);

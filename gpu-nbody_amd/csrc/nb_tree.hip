@@ -4,10 +4,12 @@
 // Only the build communicates between workgroups inside a kernel, through
 // agent-scope atomics on the child slots and the cell counter. Summarize and
 // sort run one launch per tree level, so every value they read was written by
-// an earlier launch: no fences, no waiting on other threads. (On MI355X each
-// XCD has its own L2; an agent-scope acquire/release costs an L2 invalidate /
-// write-back, which is why the per-cell hand-offs of the reference's
-// summarize and sort are replaced by level launches.)
+// an earlier launch: no fences, no waiting on other threads; the deepest
+// levels, which hold few cells, share one single-block launch with a block
+// barrier between levels. (On MI355X each XCD has its own L2; an agent-scope
+// acquire/release costs an L2 invalidate / write-back, which is why the
+// per-cell hand-offs of the reference's summarize and sort are replaced by
+// level launches.)
 #include "nb_internal.h"
 
 namespace nb {
@@ -98,11 +100,28 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
 {
     __shared__ float part[4][6];
     __shared__ int is_last;
+    // a device error stops the pipeline until the host has seen it: the steps
+    // that were already queued behind it change nothing
+    if (d.ctl->error) return;
     const int n = d.n;
     // seeded with body 0 as the reference does (boundingbox.cl:44-58)
     const float4 b0 = d.pm4[0];
     float v[6] = {b0.x, b0.y, b0.z, b0.x, b0.y, b0.z};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // four independent loads in flight per lane (one at a time leaves the
+    // read latency-bound at about a fifth of HBM bandwidth)
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = d.pm4[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[0] = fminf(v[0], q[u].x); v[1] = fminf(v[1], q[u].y); v[2] = fminf(v[2], q[u].z);
+            v[3] = fmaxf(v[3], q[u].x); v[4] = fmaxf(v[4], q[u].y); v[5] = fmaxf(v[5], q[u].z);
+        }
+    }
+    for (; i < n; i += stride) {
         const float4 p = d.pm4[i];
         v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
         v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
@@ -383,7 +402,9 @@ __global__ __launch_bounds__(256) void k_level_hist(Dev d)
     if (!ctl->error) {
         for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
             int lvl;
-            if (cell_live(d, c, ep, &lvl)) atomicAdd(&h[lvl], 1);
+            const bool live = cell_live(d, c, ep, &lvl);
+            if (live) atomicAdd(&h[lvl], 1);
+            d.clvl[c] = live ? (unsigned char)lvl : kNotLive; // for the scatter: one coalesced byte
         }
     }
     __syncthreads();
@@ -428,13 +449,12 @@ __global__ __launch_bounds__(256) void k_level_scatter(Dev d)
     if (threadIdx.x < kMaxDepth) cur[threadIdx.x] = d.lvl_blockhist[blockIdx.x * kMaxDepth + threadIdx.x];
     __syncthreads();
     if (ctl->error) return;
-    const uint32_t ep = ctl->epoch;
     const int C = min(ctl->n_cells, d.cap);
     int lo, hi;
     block_range(C, &lo, &hi);
     for (int c = lo + threadIdx.x; c < hi; c += blockDim.x) {
-        int lvl;
-        if (cell_live(d, c, ep, &lvl)) d.lvl_list[atomicAdd(&cur[lvl], 1)] = c;
+        const int lvl = d.clvl[c];
+        if (lvl != kNotLive) d.lvl_list[atomicAdd(&cur[lvl], 1)] = c;
     }
 }
 
@@ -476,20 +496,20 @@ __device__ __forceinline__ double sum8(double v)
     return v;
 }
 
+// Cell i of the level's list on the 8 lanes of one group, lane k = child slot
+// k. Every lane of the group calls it (the exchanges need all eight); lanes of
+// a group with i >= cells only take part in them.
 template <bool QUAD>
-__global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int cells)
+__device__ __forceinline__ void summarize_cell(const Dev &d, int first, int i, int k, int cells, uint32_t ep)
 {
     // products are rounded before they are summed, as in the oracle (no FMA
     // contraction: it would also fold a product into the first exchange step)
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t >> 3, k = t & 7;
-    const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
+    const bool live = i < cells;
     const int n = d.n;
-    const uint32_t ep = d.ctl->epoch;
     int c = 0, v = -1;
     if (live) {
-        c = d.lvl_list[d.ctl->lvl_start[level] + i];
+        c = d.lvl_list[first + i];
         v = dec_slot(d.slot[(size_t)c * 8 + k], ep);
     }
     double m = 0.0, x = 0.0, y = 0.0, z = 0.0;
@@ -590,6 +610,38 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
     }
 }
 
+// One level; the cell count is read on the device (a level at or below the
+// tree's depth has none), and the grid strides over the cells, so any grid
+// size is correct.
+template <bool QUAD>
+__global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level)
+{
+    const Ctl *ctl = d.ctl;
+    if (level >= ctl->levels) return;
+    const int first = ctl->lvl_start[level], cells = ctl->lvl_start[level + 1] - first;
+    const uint32_t ep = ctl->epoch;
+    const int k = threadIdx.x & 7;
+    for (int i0 = blockIdx.x * (blockDim.x >> 3); i0 < cells; i0 += gridDim.x * (blockDim.x >> 3))
+        summarize_cell<QUAD>(d, first, i0 + (threadIdx.x >> 3), k, cells, ep);
+}
+
+// The deepest levels, from the tree's last level up to `split`, in one block:
+// a barrier between levels makes each level's results visible to the next.
+// It takes every level below the launched ones, however deep the tree is.
+template <bool QUAD>
+__global__ __launch_bounds__(kTailThreads) void k_summarize_tail(Dev d, int split)
+{
+    const Ctl *ctl = d.ctl;
+    const uint32_t ep = ctl->epoch;
+    const int k = threadIdx.x & 7;
+    for (int L = ctl->levels - 1; L >= split; --L) {
+        const int first = ctl->lvl_start[L], cells = ctl->lvl_start[L + 1] - first;
+        for (int i0 = 0; i0 < cells; i0 += kTailThreads / 8)
+            summarize_cell<QUAD>(d, first, i0 + (threadIdx.x >> 3), k, cells, ep);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Stage 4: sort (sort.cl:12-76), one launch per level from the root down. A
 // cell's start arrived from its parent in the previous launch; it hands
@@ -597,18 +649,16 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level, int c
 // start, bodies their sorted position and their (x, y, z, m) in sorted order.
 // The result is the octant-path (Morton) order of the bodies.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
+__device__ __forceinline__ void sort_cell(const Dev &d, int level, int first, int i, int k, int cells)
 {
     // eight lanes per cell, one per compacted child; each lane's offset is the
     // exclusive prefix sum of its siblings' sizes (a body counts 1)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t >> 3, k = t & 7;
     const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
     const int n = d.n;
     Ctl *ctl = d.ctl;
     int c = 0, v = -1, base = 0;
     if (live) {
-        c = d.lvl_list[ctl->lvl_start[level] + i];
+        c = d.lvl_list[first + i];
         base = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
         v = d.cchild[(size_t)c * 8 + k];
     }
@@ -634,6 +684,31 @@ __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
     }
 }
 
+// One level, cell count read on the device, grid-stride (as k_summarize_level).
+__global__ __launch_bounds__(256) void k_sort_level(Dev d, int level)
+{
+    const Ctl *ctl = d.ctl;
+    if (level >= ctl->levels) return;
+    const int first = ctl->lvl_start[level], cells = ctl->lvl_start[level + 1] - first;
+    const int k = threadIdx.x & 7;
+    for (int i0 = blockIdx.x * (blockDim.x >> 3); i0 < cells; i0 += gridDim.x * (blockDim.x >> 3))
+        sort_cell(d, level, first, i0 + (threadIdx.x >> 3), k, cells);
+}
+
+// Levels `split` .. the tree's last, top-down, in one block (as k_summarize_tail).
+__global__ __launch_bounds__(kTailThreads) void k_sort_tail(Dev d, int split)
+{
+    const Ctl *ctl = d.ctl;
+    const int levels = ctl->levels;
+    const int k = threadIdx.x & 7;
+    for (int L = split; L < levels; ++L) {
+        const int first = ctl->lvl_start[L], cells = ctl->lvl_start[L + 1] - first;
+        for (int i0 = 0; i0 < cells; i0 += kTailThreads / 8)
+            sort_cell(d, L, first, i0 + (threadIdx.x >> 3), k, cells);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Stage 6: integrate (integrate.cl:17-70): kick with the acceleration just
 // computed, drift, kick. FP64 positions when NB_FP64_POSITIONS is set.
@@ -641,7 +716,7 @@ __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level, int cells)
 __global__ __launch_bounds__(256) void k_integrate(Dev d)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
+    if (i >= d.n || d.ctl->error) return;
     const float dt = d.dt;
     const float4 a = d.acc4[i], v = d.vel4[i];
     float4 p = d.pm4[i];
@@ -735,20 +810,33 @@ hipError_t launch_levels(const Dev &d, hipStream_t st)
     hipLaunchKernelGGL(k_level_scatter, dim3(kLevelBlocks), dim3(256), 0, st, d);
     return hipGetLastError();
 }
-hipError_t launch_summarize_level(const Dev &d, int level, int cells, hipStream_t st)
+// grid of a level launch: eight lanes per expected cell (at least one block)
+static dim3 level_grid(int cells_hint) { return dim3((unsigned)(((cells_hint > 1 ? cells_hint : 1) * 8 + 255) / 256)); }
+
+hipError_t launch_summarize_level(const Dev &d, int level, int cells_hint, hipStream_t st)
 {
-    if (cells <= 0) return hipSuccess;
-    const dim3 grid((cells * 8 + 255) / 256); // eight lanes per cell
     if (d.quad)
-        hipLaunchKernelGGL(k_summarize_level<true>, grid, dim3(256), 0, st, d, level, cells);
+        hipLaunchKernelGGL(k_summarize_level<true>, level_grid(cells_hint), dim3(256), 0, st, d, level);
     else
-        hipLaunchKernelGGL(k_summarize_level<false>, grid, dim3(256), 0, st, d, level, cells);
+        hipLaunchKernelGGL(k_summarize_level<false>, level_grid(cells_hint), dim3(256), 0, st, d, level);
     return hipGetLastError();
 }
-hipError_t launch_sort_level(const Dev &d, int level, int cells, hipStream_t st)
+hipError_t launch_summarize_tail(const Dev &d, int split, hipStream_t st)
 {
-    if (cells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_level, dim3((cells * 8 + 255) / 256), dim3(256), 0, st, d, level, cells);
+    if (d.quad)
+        hipLaunchKernelGGL(k_summarize_tail<true>, dim3(1), dim3(kTailThreads), 0, st, d, split);
+    else
+        hipLaunchKernelGGL(k_summarize_tail<false>, dim3(1), dim3(kTailThreads), 0, st, d, split);
+    return hipGetLastError();
+}
+hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sort_level, level_grid(cells_hint), dim3(256), 0, st, d, level);
+    return hipGetLastError();
+}
+hipError_t launch_sort_tail(const Dev &d, int split, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sort_tail, dim3(1), dim3(kTailThreads), 0, st, d, split);
     return hipGetLastError();
 }
 hipError_t launch_integrate(const Dev &d, hipStream_t st)

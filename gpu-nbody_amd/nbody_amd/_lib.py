@@ -70,7 +70,9 @@ class NbCounters(C.Structure):
         ("last_check_median", C.c_double),
         ("live_cells", C.c_int64),
         ("cell_interactions", C.c_int64),
-        ("reserved", C.c_int64 * 4),
+        ("check_ms", C.c_double),
+        ("checks", C.c_int64),
+        ("reserved", C.c_int64 * 2),
     ]
 
 

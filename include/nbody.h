@@ -163,7 +163,10 @@ typedef struct nb_counters {
     int64_t live_cells;    /* cells linked into the tree by the last build */
     int64_t cell_interactions; /* learned and coarse modes: the part of `interactions` that
                                   were cells (monopole + quadrupole) */
-    int64_t reserved[4];
+    double check_ms;       /* learned mode: device time of the sampled FP64 checks (direct sum
+                              and gather) since nb_create, cumulative */
+    int64_t checks;        /* learned mode: sampled checks run since nb_create */
+    int64_t reserved[2];
 } nb_counters;
 int nb_get_counters(nb_sim *sim, nb_counters *out);
 
