@@ -145,25 +145,28 @@ def stage_profile(sim, steps):
             c.cell_interactions / steps, int(c.live_cells))
 
 
-def cpu_baseline(state, threads):
-    """The oracle (plain-C restatement of the reference step, OpenMP force) on
-    the host cores: a full tree build of the workload plus the force pass on
-    one vote group in four, scaled to a whole step."""
-    import numpy as np
-
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     pyoracle.build()
-    pyoracle.lib().orc_set_threads(threads)
+    return pyoracle
+
+
+def _oracle_sampled_step(state, theta, softening, vote, flags, every=4):
+    """One oracle step: full bbox, build, summarize, sort and integrate, the
+    force on one vote group in `every`, scaled to all groups."""
+    import numpy as np
+
+    pyoracle = _oracle()
     n = len(state[0])
-    o = pyoracle.Oracle(n, theta=THETA_PLAIN, softening=SOFTENING, dt=DT, vote=VOTE, flags=pyoracle.FP64_POS)
+    o = pyoracle.Oracle(n, theta=theta, softening=softening, dt=DT, vote=vote, flags=flags)
     o.set_state(*state)
     t0 = time.perf_counter()
     assert o.run_stages(1, 4) == 0
     t_tree = time.perf_counter() - t0
-    groups = (n + VOTE - 1) // VOTE
-    sample = np.arange(0, groups, 4, dtype=np.int32)
+    groups = (n + vote - 1) // vote
+    sample = np.arange(0, groups, every, dtype=np.int32)
     t0 = time.perf_counter()
     assert o.force_groups(sample) == 0
     t_force = (time.perf_counter() - t0) * groups / len(sample)
@@ -171,12 +174,81 @@ def cpu_baseline(state, threads):
     o.run_stages(6, 6)
     t_int = time.perf_counter() - t0
     o.close()
+    return t_tree, t_force, t_int, len(sample), groups
+
+
+def cpu_baseline(state, threads):
+    """The oracle (plain-C restatement of the reference step, OpenMP force) on
+    the host cores: a full tree build of the workload plus the force pass on
+    one vote group in four, scaled to a whole step."""
+    pyoracle = _oracle()
+    pyoracle.lib().orc_set_threads(threads)
+    n = len(state[0])
+    t_tree, t_force, t_int, ns, groups = _oracle_sampled_step(state, THETA_PLAIN, SOFTENING, VOTE,
+                                                              pyoracle.FP64_POS)
     step_s = t_tree + t_force + t_int
     return {"value": round(1.0 / step_s, 6), "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": (f"oracle/bh_oracle.c (reference step restated in C, OpenMP force), plain theta "
                        f"{THETA_PLAIN}, vote {VOTE}, N={n}: full bbox+build+summarize+sort "
                        f"({t_tree:.2f} s) and integrate ({t_int:.2f} s), force on 1 of every 4 vote groups "
-                       f"({len(sample)} groups) scaled x{groups / len(sample):.0f} to {t_force:.1f} s")}
+                       f"({ns} groups) scaled x{groups / ns:.0f} to {t_force:.1f} s")}
+
+
+def config_table(threads):
+    """The other configurations of the benchmark description with plain
+    Barnes-Hut, each timed on the GPU (HIP engine) and on the host cores (the
+    oracle): C1 in full (10 steps), one step of C2, and one step of C3 at
+    theta 0.4 with the force on one vote group in four, scaled."""
+    import nbody_amd
+    from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
+    from nbody_amd import universe as U
+    from nbody_amd.universe import ArrayUniverse
+
+    pyoracle = _oracle()
+    pyoracle.lib().orc_set_threads(threads)
+    nbody_amd.load()
+    cfgs = [
+        ("C1", "uniform ball in rotation, N=16384", lambda: U.uniform_ball(16384, seed=1),
+         dict(theta=0.5, softening=0.05, vote_width=16, flags=0), 10, 1),
+        ("C2", "Plummer sphere, N=1048576", lambda: U.plummer(1 << 20, seed=2),
+         dict(theta=0.5, softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE), 20, 3),
+        ("C3", "two Plummer spheres, N=4194304, FP64 positions", lambda: U.two_plummer(1 << 22, seed=3),
+         dict(theta=0.4, softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE | _lib.NB_FP64_POSITIONS),
+         20, 3),
+    ]
+    out = {}
+    for name, desc, gen, kw, steps, warm in cfgs:
+        st = gen()
+        n = len(st[0])
+        sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, n, ArrayUniverse(st), dt=DT, **kw)
+        sim.init(None)
+        sim.steps(warm)
+        t0 = time.perf_counter()
+        sim.steps(steps)
+        gpu_s = (time.perf_counter() - t0) / steps
+        sim.close()
+        oflags = pyoracle.FP64_POS if kw["flags"] & _lib.NB_FP64_POSITIONS else 0
+        if name == "C1":
+            o = pyoracle.Oracle(n, theta=kw["theta"], softening=kw["softening"], dt=DT,
+                                vote=kw["vote_width"], flags=oflags)
+            o.set_state(*st)
+            t0 = time.perf_counter()
+            assert o.step(steps) == 0
+            cpu_s = (time.perf_counter() - t0) / steps
+            o.close()
+            how = f"{steps} full steps"
+        else:
+            every = 1 if name == "C2" else 4
+            t_tree, t_force, t_int, ns, groups = _oracle_sampled_step(st, kw["theta"], kw["softening"],
+                                                                      kw["vote_width"], oflags, every)
+            cpu_s = t_tree + t_force + t_int
+            how = "1 full step" if every == 1 else f"1 step, force on {ns} of {groups} vote groups scaled"
+        out[name] = {"workload": desc, "theta": kw["theta"], "softening": kw["softening"],
+                     "vote_width": kw["vote_width"], "gpu_ms_per_step": round(gpu_s * 1e3, 3),
+                     "gpu_steps_timed": steps, "cpu_s_per_step": round(cpu_s, 4), "cpu_sample": how,
+                     "gpu_over_cpu": round(cpu_s / gpu_s, 1)}
+        log(f"[{name}] GPU {gpu_s * 1e3:.3f} ms/step, CPU {cpu_s:.3f} s/step ({how})")
+    return out
 
 
 def main():
@@ -188,6 +260,8 @@ def main():
     ap.add_argument("--mode", choices=["learned", "plain"], default=None)
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-config-table", action="store_true",
+                    help="skip the C1-C3 table (GPU and CPU seconds per step of the other configurations)")
     ap.add_argument("--alpha", type=float, default=None,
                     help="use this relative-test alpha instead of calibrating one")
     ap.add_argument("--train-iters", type=int, default=None, help="Adam iterations (default 4000)")
@@ -333,6 +407,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
         line["cpu_baseline"] = cpu_baseline(state, threads)
+        if not args.no_config_table:
+            line["cpu_baseline"]["configs"] = config_table(threads)
     print(json.dumps(line), flush=True)
 
 

@@ -107,22 +107,29 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
     // seeded with body 0 as the reference does (boundingbox.cl:44-58)
     const float4 b0 = d.pm4[0];
     float v[6] = {b0.x, b0.y, b0.z, b0.x, b0.y, b0.z};
-    // four independent loads in flight per lane (one at a time leaves the
-    // read latency-bound at about a fifth of HBM bandwidth)
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
+    // Four independent loads in flight per lane (one at a time leaves the
+    // read latency-bound at about a fifth of HBM bandwidth), from one
+    // contiguous 4 KiB span per wave and iteration: body j is read by wave
+    // (j / 256) mod W at element j mod 256, so the reads in flight stay one
+    // contiguous region (four loads a whole grid stride apart hit the same
+    // HBM channels and ran slower than one).
+    const int lane = threadIdx.x & 63;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    long long i = (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 256 + lane;
+    for (; i + 192 < n; i += nwaves * 256) {
         float4 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = d.pm4[i + u * stride];
+        for (int u = 0; u < 4; ++u) q[u] = d.pm4[i + 64 * u];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             v[0] = fminf(v[0], q[u].x); v[1] = fminf(v[1], q[u].y); v[2] = fminf(v[2], q[u].z);
             v[3] = fmaxf(v[3], q[u].x); v[4] = fmaxf(v[4], q[u].y); v[5] = fmaxf(v[5], q[u].z);
         }
     }
-    for (; i < n; i += stride) {
-        const float4 p = d.pm4[i];
+    // the span that crosses n (every later span of this wave starts past n)
+    for (int u = 0; u < 4; ++u) {
+        if (i + 64 * u >= n) break;
+        const float4 p = d.pm4[i + 64 * u];
         v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
         v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
     }
