@@ -97,6 +97,27 @@ __device__ __forceinline__ float gelu_grad(float x)
 
 __device__ __forceinline__ float clipz(float z) { return fminf(fmaxf(z, -kClipZ), kClipZ); }
 
+// Inference GELU in the logistic form of the same function,
+// 0.5 x (1 + tanh(u)) = x / (1 + exp(-2 u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// with the hardware 2^x and reciprocal: 7 instructions. Limits: x -> +inf
+// gives x, x -> -inf gives -0 (2^+inf = inf, 1 / inf = 0).
+constexpr float kGeluK1 = -2.0f * kGeluC * 1.4426950408889634f;          // -2 c log2(e)
+constexpr float kGeluK2 = -2.0f * kGeluC * kGeluA * 1.4426950408889634f; // -2 c a log2(e)
+
+__device__ __forceinline__ float gelu_infer(float x)
+{
+    const float s = fmaf(kGeluK2, x * x, kGeluK1);
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(s * x));
+}
+
+// two floats to packed BF16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
+{
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 h = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, h);
+}
+
 __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t *p)
 {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(p));
@@ -115,15 +136,11 @@ static_assert(kStackC >= kStack && kStackC * 4 >= 64 * kFeat * 2, "coarse stack 
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bf16x8 pack_gelu(f32x4 lo, float4 blo, f32x4 hi, float4 bhi)
 {
-    u16x8 h;
-    h[0] = to_bf16(gelu(lo[0] + blo.x));
-    h[1] = to_bf16(gelu(lo[1] + blo.y));
-    h[2] = to_bf16(gelu(lo[2] + blo.z));
-    h[3] = to_bf16(gelu(lo[3] + blo.w));
-    h[4] = to_bf16(gelu(hi[0] + bhi.x));
-    h[5] = to_bf16(gelu(hi[1] + bhi.y));
-    h[6] = to_bf16(gelu(hi[2] + bhi.z));
-    h[7] = to_bf16(gelu(hi[3] + bhi.w));
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 h = {pack_bf16(gelu_infer(lo[0] + blo.x), gelu_infer(lo[1] + blo.y)),
+                     pack_bf16(gelu_infer(lo[2] + blo.z), gelu_infer(lo[3] + blo.w)),
+                     pack_bf16(gelu_infer(hi[0] + bhi.x), gelu_infer(hi[1] + bhi.y)),
+                     pack_bf16(gelu_infer(hi[2] + bhi.z), gelu_infer(hi[3] + bhi.w))};
     return __builtin_bit_cast(bf16x8, h);
 }
 
@@ -378,34 +395,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             gby += __shfl_xor(gby, o);
             gbz += __shfl_xor(gbz, o);
         }
-        gx /= nv;
-        gy /= nv;
-        gz /= nv;
-        gax /= nv;
-        gay /= nv;
-        gaz /= nv;
-        gbx /= nv;
-        gby /= nv;
-        gbz /= nv;
-        gqx /= nv;
-        gqy /= nv;
-        gqz /= nv;
+        // the features end as BF16 network inputs: hardware reciprocal,
+        // square root and logarithm (about 1 ulp) instead of the IEEE sequences
+        const float inv_nv = __builtin_amdgcn_rcpf(nv);
+        gx *= inv_nv;
+        gy *= inv_nv;
+        gz *= inv_nv;
+        gax *= inv_nv;
+        gay *= inv_nv;
+        gaz *= inv_nv;
+        gbx *= inv_nv;
+        gby *= inv_nv;
+        gbz *= inv_nv;
+        gqx *= inv_nv;
+        gqy *= inv_nv;
+        gqz *= inv_nv;
         const float ox = me.x - gx, oy = me.y - gy, oz = me.z - gz;
-        float rg = valid ? sqrtf(ox * ox + oy * oy + oz * oz) : 0.f;
+        float rg = valid ? __builtin_amdgcn_sqrtf(ox * ox + oy * oy + oz * oz) : 0.f;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) rg = fmaxf(rg, __shfl_xor(rg, o));
 
         float f[kFeat];
 #pragma unroll
         for (int i = 0; i < kFeat; ++i) f[i] = 0.f;
-        const float A = sqrtf(ax * ax + ay * ay + az * az);
+        const float A = __builtin_amdgcn_sqrtf(ax * ax + ay * ay + az * az);
         float e2x = 0.f, e2y = 0.f, e2z = 0.f;
         if (A > 0.f) {
-            const float inv = 1.0f / A;
+            const float inv = __builtin_amdgcn_rcpf(A);
             const float e1x = ax * inv, e1y = ay * inv, e1z = az * inv;
             const float qe1 = qx * e1x + qy * e1y + qz * e1z;
             float wx = qx - qe1 * e1x, wy = qy - qe1 * e1y, wz = qz - qe1 * e1z;
-            float W = sqrtf(wx * wx + wy * wy + wz * wz);
+            float W = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
             if (!(W > 0.f)) {
                 // no quadrupole direction: the coordinate axis least aligned with e1
                 const float fx = fabsf(e1x), fy = fabsf(e1y), fz = fabsf(e1z);
@@ -416,11 +436,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                 wx = tx - te1 * e1x;
                 wy = ty - te1 * e1y;
                 wz = tz - te1 * e1z;
-                W = sqrtf(wx * wx + wy * wy + wz * wz);
+                W = __builtin_amdgcn_sqrtf(wx * wx + wy * wy + wz * wz);
             }
-            e2x = wx / W;
-            e2y = wy / W;
-            e2z = wz / W;
+            const float iW = __builtin_amdgcn_rcpf(W);
+            e2x = wx * iW;
+            e2y = wy * iW;
+            e2z = wz * iW;
             const float e3x = e1y * e2z - e1z * e2y;
             const float e3y = e1z * e2x - e1x * e2z;
             const float e3z = e1x * e2y - e1y * e2x;
@@ -433,24 +454,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             f[4] = (qx * e2x + qy * e2y + qz * e2z) * inv;
             NB_LOC(ax - gax, ay - gay, az - gaz, 5);
             if (rg > 0.f) {
-                const float ir = 1.0f / rg;
+                const float ir = __builtin_amdgcn_rcpf(rg);
                 f[8] = (ox * e1x + oy * e1y + oz * e1z) * ir;
                 f[9] = (ox * e2x + oy * e2y + oz * e2z) * ir;
                 f[10] = (ox * e3x + oy * e3y + oz * e3z) * ir;
             }
             NB_LOC(gbx, gby, gbz, 11);
             NB_LOC(gqx, gqy, gqz, 14);
-            f[17] = log1pf((float)nc);
-            f[18] = log1pf((float)nbod);
-            const float qn = sqrtf(qx * qx + qy * qy + qz * qz);
-            const float bn = sqrtf(bx * bx + by * by + bz * bz);
-            f[19] = logf(fmaxf(qn * inv, 1e-30f));
-            f[20] = logf(fmaxf(bn * inv, 1e-30f));
-            f[21] = (smc + smb) > 0.f ? smc / (smc + smb) : 0.f;
-            f[22] = logf(fmaxf(rg / root_r, 1e-30f));
+            f[17] = __logf(1.0f + (float)nc); // counts below 2^24: 1 + n is exact
+            f[18] = __logf(1.0f + (float)nbod);
+            const float qn = __builtin_amdgcn_sqrtf(qx * qx + qy * qy + qz * qz);
+            const float bn = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
+            f[19] = __logf(fmaxf(qn * inv, 1e-30f));
+            f[20] = __logf(fmaxf(bn * inv, 1e-30f));
+            f[21] = (smc + smb) > 0.f ? smc * __builtin_amdgcn_rcpf(smc + smb) : 0.f;
+            f[22] = __logf(fmaxf(rg * __builtin_amdgcn_rcpf(root_r), 1e-30f));
             const float mx = ax - bx - qx, my = ay - by - qy, mz = az - bz - qz;
-            const float mn = sqrtf(mx * mx + my * my + mz * mz);
-            f[23] = (mn > 0.f && bn > 0.f) ? (mx * bx + my * by + mz * bz) / (mn * bn) : 0.f;
+            const float mn = __builtin_amdgcn_sqrtf(mx * mx + my * my + mz * mz);
+            f[23] = (mn > 0.f && bn > 0.f) ? (mx * bx + my * by + mz * bz) * __builtin_amdgcn_rcpf(mn * bn) : 0.f;
 #undef NB_LOC
         }
         if constexpr (OUT == kOutRaw) {
@@ -469,9 +490,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             uint32_t w[kFeat / 2];
 #pragma unroll
             for (int i = 0; i < kFeat; i += 2) {
-                const uint32_t lo = to_bf16(clipz((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]));
-                const uint32_t hi = to_bf16(clipz((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]));
-                w[i / 2] = valid ? (lo | (hi << 16)) : 0u;
+                const uint32_t h = pack_bf16(clipz((f[i] - L.mlp.xmu[i]) * L.mlp.xrs[i]),
+                                             clipz((f[i + 1] - L.mlp.xmu[i + 1]) * L.mlp.xrs[i + 1]));
+                w[i / 2] = valid ? h : 0u;
             }
             uint4 *o = reinterpret_cast<uint4 *>(fs + lane * kFeat);
             o[0] = make_uint4(w[0], w[1], w[2], w[3]);
@@ -484,7 +505,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             // residual = |a| (y0 e1 + y1 e2 + y2 e3)
             float rx = ax, ry = ay, rz = az;
             if (A > 0.f) {
-                const float e1x = ax / A, e1y = ay / A, e1z = az / A;
+                const float iA = __builtin_amdgcn_rcpf(A);
+                const float e1x = ax * iA, e1y = ay * iA, e1z = az * iA;
                 const float e3x = e1y * e2z - e1z * e2y, e3y = e1z * e2x - e1x * e2z, e3z = e1x * e2y - e1y * e2x;
                 rx += A * (y0 * e1x + y1 * e2x + y2 * e3x);
                 ry += A * (y0 * e1y + y1 * e2y + y2 * e3y);
