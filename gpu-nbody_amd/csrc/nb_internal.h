@@ -16,12 +16,15 @@
 //            cdata : float4[8*cap]  (x, y, z, m) of those children: body position
 //                                   and mass, or the child cell's FP32 centre of
 //                                   mass and mass -- a popped cell is one
-//                                   contiguous 160-byte read in the traversal
+//                                   contiguous 160-byte read in the traversal;
+//                                   slots past the last child are not written
 //            cq64  : double[8*cap]  second moments about the centre of mass, learned
 //                                   mode only: traceless Q (xx, yy, zz, xy, xz, yz),
 //                                   trace T, 0
-//            cqa/cqb: float4[8*cap] each child's (Qxx, Qyy, Qzz, T eps^2) and
-//                                   (Qxy, Qxz, Qyz, 0), next to cdata
+//            cqa/cqb: float4[8*cap] each child cell's (Qxx, Qyy, Qzz, T eps^2) and
+//                                   (Qxy, Qxz, Qyz, 0), next to cdata; with cmac
+//                                   (float2[8*cap], the opening-test constants)
+//                                   written for cell children only
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
 //            sorted : int32[N]      body index at each sorted position

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f; // total, Kahan
     float bx = 0.f, by = 0.f, bz = 0.f;                                // bodies
     float qx = 0.f, qy = 0.f, qz = 0.f;                                // quadrupole part
-    float smc = 0.f, smb = 0.f;
+    float smb = 0.f; // mass of the bodies met (the accepted cells hold the rest)
     int nc = 0, nbod = 0;
     long long ninter = 0, ncell = 0;
     bool bad = false;
@@ -332,7 +332,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
                     pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
                     pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
                     pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
-                    smc += q[k].w;
                     ++nc;
                 } else {
                     // monopole and quadrupole radial terms share one factor
@@ -467,7 +466,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float bn = __builtin_amdgcn_sqrtf(bx * bx + by * by + bz * bz);
             f[19] = __logf(fmaxf(qn * inv, 1e-30f));
             f[20] = __logf(fmaxf(bn * inv, 1e-30f));
-            f[21] = (smc + smb) > 0.f ? smc * __builtin_amdgcn_rcpf(smc + smb) : 0.f;
+            // accepted-cell mass over the mass met: every body is met once, as a
+            // body or inside an accepted cell, so the cells hold the total
+            // mass (the root's) minus the bodies' (one sum fewer in the loop)
+            const float mtot = (float)d.com64[0].w;
+            f[21] = mtot > 0.f ? (mtot - smb) * __builtin_amdgcn_rcpf(mtot) : 0.f;
             f[22] = __logf(fmaxf(rg * __builtin_amdgcn_rcpf(root_r), 1e-30f));
             const float mx = ax - bx - qx, my = ay - by - qy, mz = az - bz - qz;
             const float mn = __builtin_amdgcn_sqrtf(mx * mx + my * my + mz * mz);

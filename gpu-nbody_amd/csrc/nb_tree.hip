@@ -562,9 +562,12 @@ __device__ __forceinline__ void summarize_cell(const Dev &d, int first, int i, i
     // reference's 0 * inf = NaN (summarizetree.cl:161-167)
     const double inv = sm > 0.0 ? 1.0 / sm : 0.0;
     const double4 com = make_double4(sx * inv, sy * inv, sz * inv, sm);
+    // the traversals and the sort read a slot's record only up to the first
+    // empty slot: the empty ones keep stale records (a cell has 3 children on
+    // average, so this skips well over half of the record writes)
     if (live) {
         d.cchild[(size_t)c * 8 + at] = v;
-        d.cdata[(size_t)c * 8 + at] = rec;
+        if (v >= 0) d.cdata[(size_t)c * 8 + at] = rec;
         if (k == 0) {
             d.com64[c] = com;
             d.count[c] = scnt;
@@ -603,11 +606,16 @@ __device__ __forceinline__ void summarize_cell(const Dev &d, int first, int i, i
 #pragma unroll
         for (int j = 0; j < 7; ++j) t7[j] = sum8(t7[j]);
         if (live) {
-            const double eps2 = (double)d.eps2;
-            d.cqa[(size_t)c * 8 + at] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[6] * eps2));
-            d.cqb[(size_t)c * 8 + at] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
-            const double mb3 = m * ((bc * bc) * bc);
-            d.cmac[(size_t)c * 8 + at] = make_float2(fmaf((float)(bc * bc), d.it2_coarse, d.eps2), (float)(mb3 * mb3));
+            // read for cell children only (a body meets no opening test and
+            // has no second moments)
+            if (v >= n) {
+                const double eps2 = (double)d.eps2;
+                d.cqa[(size_t)c * 8 + at] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)(q[6] * eps2));
+                d.cqb[(size_t)c * 8 + at] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
+                const double mb3 = m * ((bc * bc) * bc);
+                d.cmac[(size_t)c * 8 + at] =
+                    make_float2(fmaf((float)(bc * bc), d.it2_coarse, d.eps2), (float)(mb3 * mb3));
+            }
             if (k == 0) {
                 double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);
                 cq[0] = make_double4(t7[0], t7[1], t7[2], t7[3]);
