@@ -140,7 +140,6 @@ nb::Ctl initial_ctl(const nb_sim *s, uint32_t epoch)
     c.n_cells = 1;
     c.max_depth = 1; // GPUBarnesHutNBodySimulation.java:170
     c.error = 0;
-    c.bbox_count = 0;
     c.interactions = 0;
     c.cell_interactions = 0;
     (void)s;
@@ -609,7 +608,7 @@ int nb_create(const nb_params *p, nb_sim **out)
     if (d.quad && ensure_learn(s) != NB_OK) return bail(NB_E_HIP);
     // epoch 0 tags every slot as stale; the first bounding box moves to epoch 1
     ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int2), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int4), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.count, 0, cap * sizeof(int), s->stream) == hipSuccess;
     nb::Ctl c = initial_ctl(s, 0);
     ok = ok && hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream) == hipSuccess;

@@ -6,7 +6,7 @@
 //            acc4  : float4[N]   (ax, ay, az, 0)
 //            pos64 : double4[N]  FP64 positions (NB_FP64_POSITIONS only)
 //   octree   slot  : uint64[8*cap]  child slots, (epoch << 32) | child
-//            cinfo : int2[cap]      (parent, level | octant << 8), written when
+//            cinfo : int4[cap]      (parent, level | octant << 8, epoch, 0), written when
 //                                   the cell is published
 //            count : int32[cap]     bodies in the cell's subtree
 //            start : int32[cap]     first sorted position of the subtree
@@ -68,7 +68,7 @@ struct Ctl {
     int n_cells;         // cell allocation counter (reference _bottom)
     int max_depth;       // running maximum (reference _maxDepth, java:170)
     int error;           // DevError
-    uint32_t bbox_count; // last-block detection (reference _blockCount)
+    uint32_t unused0;
     float radius;        // reference _radius
     float root[3];
     float dq[kMaxDepth]; // opening thresholds (calculateforce.cl:52-67)
@@ -91,7 +91,7 @@ struct Dev {
     float4 *pm4, *vel4, *acc4;
     double4 *pos64;
     unsigned long long *slot;
-    int2 *cinfo;
+    int4 *cinfo;
     int *count, *start;
     double4 *com64;
     int *cchild;

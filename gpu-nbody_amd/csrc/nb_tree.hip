@@ -40,26 +40,29 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int code)
                                          __ATOMIC_RELAXED, AGENT);
 }
 
-// A cell is in the tree when the slot its publisher recorded still names it.
+// A cell is in the tree when it was published in this tree's epoch: a
+// published cell is never unlinked (a slot that holds a cell never changes
+// within a step), and its record is written right after the publishing CAS.
+// Cells a thread took from the pool but never published keep an older epoch.
 __device__ __forceinline__ bool cell_live(const Dev &d, int c, uint32_t ep, int *level)
 {
     if (c == 0) {
         *level = 0;
         return true;
     }
-    const int2 info = d.cinfo[c];
-    const int p = info.x;
-    if (p < 0 || p >= d.cap) return false;
-    const int oct = (info.y >> 8) & 7;
+    const int4 info = d.cinfo[c];
     *level = info.y & 255;
-    return dec_slot(d.slot[(size_t)p * 8 + oct], ep) == d.n + c && *level < kMaxDepth;
+    return (uint32_t)info.z == ep && *level < kMaxDepth;
 }
 
 // ---------------------------------------------------------------------------
-// Stage 1: bounding box (boundingbox.cl:20-199). Grid-stride min/max, wave64
-// shuffle reduction, one partial per block, and the last block to arrive
+// Stage 1: bounding box (boundingbox.cl:20-199). Two launches: min/max per
+// block (wave64 shuffle reduction, one partial per block), then one block
 // combines the partials and sets up the root, the opening table and the
-// per-step counters.
+// per-step counters. The launch boundary orders the two, where the
+// reference's last-block hand-off (boundingbox.cl:133-196) needs an
+// agent-scope release per block: on MI355X an L2 write-back each, which cost
+// about 40 us per step at 4M bodies against 11 us for the reads.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float wave_min(float v)
 {
@@ -96,10 +99,15 @@ __device__ void block_minmax(float v[6], float (*part)[6])
     }
 }
 
+__device__ __forceinline__ void minmax_into(float v[6], float4 q)
+{
+    v[0] = fminf(v[0], q.x); v[1] = fminf(v[1], q.y); v[2] = fminf(v[2], q.z);
+    v[3] = fmaxf(v[3], q.x); v[4] = fmaxf(v[4], q.y); v[5] = fmaxf(v[5], q.z);
+}
+
 __global__ __launch_bounds__(256) void k_bbox(Dev d)
 {
     __shared__ float part[4][6];
-    __shared__ int is_last;
     // a device error stops the pipeline until the host has seen it: the steps
     // that were already queued behind it change nothing
     if (d.ctl->error) return;
@@ -107,12 +115,9 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
     // seeded with body 0 as the reference does (boundingbox.cl:44-58)
     const float4 b0 = d.pm4[0];
     float v[6] = {b0.x, b0.y, b0.z, b0.x, b0.y, b0.z};
-    // Four independent loads in flight per lane (one at a time leaves the
-    // read latency-bound at about a fifth of HBM bandwidth), from one
-    // contiguous 4 KiB span per wave and iteration: body j is read by wave
-    // (j / 256) mod W at element j mod 256, so the reads in flight stay one
-    // contiguous region (four loads a whole grid stride apart hit the same
-    // HBM channels and ran slower than one).
+    // Four independent loads in flight per lane, from one contiguous 4 KiB
+    // span per wave and iteration: body j is read by wave (j / 256) mod W at
+    // element j mod 256.
     const int lane = threadIdx.x & 63;
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     long long i = (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 256 + lane;
@@ -121,39 +126,35 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
 #pragma unroll
         for (int u = 0; u < 4; ++u) q[u] = d.pm4[i + 64 * u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            v[0] = fminf(v[0], q[u].x); v[1] = fminf(v[1], q[u].y); v[2] = fminf(v[2], q[u].z);
-            v[3] = fmaxf(v[3], q[u].x); v[4] = fmaxf(v[4], q[u].y); v[5] = fmaxf(v[5], q[u].z);
-        }
+        for (int u = 0; u < 4; ++u) minmax_into(v, q[u]);
     }
     // the span that crosses n (every later span of this wave starts past n)
     for (int u = 0; u < 4; ++u) {
         if (i + 64 * u >= n) break;
-        const float4 p = d.pm4[i + 64 * u];
-        v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
-        v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
+        minmax_into(v, d.pm4[i + 64 * u]);
     }
     block_minmax(v, part);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) d.bbox_part[6 * blockIdx.x + k] = v[k];
-        const uint32_t prev = __hip_atomic_fetch_add(&d.ctl->bbox_count, 1u, __ATOMIC_ACQ_REL, AGENT);
-        is_last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// one block: the partials of k_bbox's `blocks` blocks
+__global__ __launch_bounds__(256) void k_bbox_final(Dev d, int blocks)
+{
+    __shared__ float part[4][6];
+    Ctl *c = d.ctl;
+    if (c->error) return;
     float w[6] = {d.bbox_part[0], d.bbox_part[1], d.bbox_part[2], d.bbox_part[3], d.bbox_part[4],
                   d.bbox_part[5]};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
         const float *q = d.bbox_part + 6 * b;
         w[0] = fminf(w[0], q[0]); w[1] = fminf(w[1], q[1]); w[2] = fminf(w[2], q[2]);
         w[3] = fmaxf(w[3], q[3]); w[4] = fmaxf(w[4], q[4]); w[5] = fmaxf(w[5], q[5]);
     }
-    __syncthreads();
     block_minmax(w, part);
     if (threadIdx.x == 0) {
-        Ctl *c = d.ctl;
         // root cell, boundingbox.cl:171-189
         c->root[0] = 0.5f * (w[0] + w[3]);
         c->root[1] = 0.5f * (w[1] + w[4]);
@@ -171,7 +172,6 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d)
         c->n_cells = 1; // the root is cell 0 (reference: _bottom = NUMBER_OF_NODES, :180)
         c->epoch += 1;  // every child slot of the last tree turns stale
         c->step += 1;   // boundingbox.cl:195
-        c->bbox_count = 0;
     }
 }
 
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int
                     if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
                     if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELAXED,
                                                              __ATOMIC_RELAXED, AGENT)) {
-                        d.cinfo[c] = make_int2(p, (level + 1) | (oct << 8));
+                        d.cinfo[c] = make_int4(p, (level + 1) | (oct << 8), (int)ep, 0);
                         // cur goes on later from c; the displaced body v first
                         if (npend >= kPending) { set_error(ctl, kErrSync); failed = true; break; }
                         pend_body[npend][tid] = cur;
@@ -805,6 +805,7 @@ __global__ __launch_bounds__(256) void k_paths(Dev d, unsigned long long *paths3
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, d, blocks);
     return hipGetLastError();
 }
 hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
