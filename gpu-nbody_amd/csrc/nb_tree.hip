@@ -266,13 +266,9 @@ __global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int
     unsigned long long po0 = 0, po1 = 0, po2 = 0; // octants of the cached path
     int plen = 0; // pc[1 .. plen-1] hold the cells of the cached path
     for (int i = first; i < last && !failed; i += step) {
-        int b;
-        if (seed) {
-            b = d.sorted[i * stride];
-        } else {
-            if (i % stride == 0 && i / stride < nseed) continue; // placed by the seed launch
-            b = d.sorted[i];
-        }
+        if (!seed && i % stride == 0 && i / stride < nseed) continue; // placed by the seed launch
+        const int si = seed ? i * stride : i;
+        const int b = d.sorted[si];
         int cur = b, npend = 0;
         float4 me = d.pm4[cur];
         int p = 0, level = 0;
@@ -418,35 +414,55 @@ __global__ __launch_bounds__(256) void k_level_hist(Dev d)
     if (threadIdx.x < kMaxDepth) d.lvl_blockhist[blockIdx.x * kMaxDepth + threadIdx.x] = h[threadIdx.x];
 }
 
-// one block of kMaxDepth threads: thread L turns the column of level L into
-// global list offsets and totals
-__global__ __launch_bounds__(kMaxDepth) void k_level_scan(Dev d)
+// One block of 1024 threads turns the per-block level histograms into list
+// offsets: thread (seg, L) owns level L in block segment seg (16 segments of
+// kLevelBlocks / 16 blocks); segment sums, their prefix per level, the level
+// starts, then each thread writes its segment's offsets.
+constexpr int kScanSegs = 1024 / kMaxDepth;
+static_assert(kLevelBlocks % kScanSegs == 0, "level blocks split evenly into scan segments");
+
+__global__ __launch_bounds__(1024) void k_level_scan(Dev d)
 {
-    __shared__ int tot[kMaxDepth];
-    const int L = threadIdx.x;
-    int s = 0;
-    for (int b = 0; b < kLevelBlocks; ++b) {
-        const int v = d.lvl_blockhist[b * kMaxDepth + L];
-        d.lvl_blockhist[b * kMaxDepth + L] = s;
-        s += v;
-    }
-    tot[L] = s;
+    __shared__ int segsum[kScanSegs][kMaxDepth];
+    __shared__ int base[kMaxDepth];
+    const int L = threadIdx.x % kMaxDepth, seg = threadIdx.x / kMaxDepth;
+    constexpr int per = kLevelBlocks / kScanSegs;
+    int *h = d.lvl_blockhist + (size_t)seg * per * kMaxDepth + L;
+    int sum = 0;
+    for (int b = 0; b < per; ++b) sum += h[b * kMaxDepth];
+    segsum[seg][L] = sum;
     __syncthreads();
-    if (L == 0) {
+    if (seg == 0) { // exclusive prefix over the segments of level L
+        int acc = 0;
+        for (int k = 0; k < kScanSegs; ++k) {
+            const int v = segsum[k][L];
+            segsum[k][L] = acc;
+            acc += v;
+        }
+        base[L] = acc; // level total, for the moment
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         Ctl *ctl = d.ctl;
         int acc = 0, levels = 0;
         for (int k = 0; k < kMaxDepth; ++k) {
+            const int t = base[k];
             ctl->lvl_start[k] = acc;
-            acc += tot[k];
-            if (tot[k] > 0) levels = k + 1;
+            base[k] = acc;
+            acc += t;
+            if (t > 0) levels = k + 1;
         }
         ctl->lvl_start[kMaxDepth] = acc;
         ctl->levels = levels;
         ctl->live_cells = acc;
     }
     __syncthreads();
-    const int base = d.ctl->lvl_start[L];
-    for (int b = 0; b < kLevelBlocks; ++b) d.lvl_blockhist[b * kMaxDepth + L] += base;
+    int run = base[L] + segsum[seg][L];
+    for (int b = 0; b < per; ++b) {
+        const int v = h[b * kMaxDepth];
+        h[b * kMaxDepth] = run;
+        run += v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_level_scatter(Dev d)
@@ -822,7 +838,7 @@ hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
 hipError_t launch_levels(const Dev &d, hipStream_t st)
 {
     hipLaunchKernelGGL(k_level_hist, dim3(kLevelBlocks), dim3(256), 0, st, d);
-    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(kMaxDepth), 0, st, d);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1024), 0, st, d);
     hipLaunchKernelGGL(k_level_scatter, dim3(kLevelBlocks), dim3(256), 0, st, d);
     return hipGetLastError();
 }
