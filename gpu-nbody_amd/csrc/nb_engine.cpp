@@ -54,9 +54,15 @@ struct nb_sim {
     long long steps_since_model = 0;
     uint64_t check_seed = 0x5eed;
     float4 *chk_acc = nullptr;
-    hipEvent_t chk_ev[2] = {nullptr, nullptr}; // bracket the sampled check on the stream
+    hipEvent_t chk_ev[2] = {nullptr, nullptr}; // bracket the sampled check's direct sum
     double check_ms = 0.0;                     // device time of the sampled checks so far
     long long checks = 0;
+    // the direct sum of a check step runs on a second stream (default
+    // priority) beside the step's tree stages and traversal
+    hipStream_t side = nullptr;
+    hipEvent_t chk_go = nullptr; // the step's positions are final (recorded on stream)
+    int32_t *chk_host = nullptr; // pinned sample indices, so the upload never blocks the host
+    int chk_host_cap = 0, chk_ns = 0;
     std::string err;
 };
 
@@ -384,6 +390,9 @@ void free_all(nb_sim *s)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : s->chk_ev)
         if (e) (void)hipEventDestroy(e);
+    if (s->chk_go) (void)hipEventDestroy(s->chk_go);
+    if (s->chk_host) (void)hipHostFree(s->chk_host);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -459,37 +468,61 @@ uint64_t splitmix64(uint64_t &x)
     return z ^ (z >> 31);
 }
 
-// Force stage of a checked learned step: the learned forces are stored
-// (velocity correction included), check_samples bodies drawn with replacement
-// are compared with the FP64 direct sum, and if the median relative error is
-// above check_tol the step's forces are recomputed with plain theta. The
-// second pass corrects the velocities by (a_plain - a_learned) dt / 2, so the
-// step ends exactly as a plain step would. The fallback holds until a new
-// model is installed.
+// Starts a check step's FP64 direct sum on the side stream. It reads only the
+// positions at the start of the step, which nothing changes before the
+// integrate, so it runs beside the tree stages and the traversal; the step
+// joins it (an event) before it compares.
+int start_check(nb_sim *s)
+{
+    const int n = s->d.n;
+    const int ns = std::min(s->p.check_samples, n);
+    int r = ensure_check_buffers(s, ns);
+    if (r) return r;
+    if (!s->side) HIP_TRY(s, hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    for (auto &ev : s->chk_ev)
+        if (!ev) HIP_TRY(s, hipEventCreate(&ev));
+    if (!s->chk_go) HIP_TRY(s, hipEventCreateWithFlags(&s->chk_go, hipEventDisableTiming));
+    if (s->chk_host_cap < ns) {
+        // no upload from it is in flight: the last check step waited for its own
+        if (s->chk_host) (void)hipHostFree(s->chk_host);
+        s->chk_host = nullptr;
+        s->chk_host_cap = 0;
+        HIP_TRY(s, hipHostMalloc((void **)&s->chk_host, (size_t)ns * sizeof(int32_t)));
+        s->chk_host_cap = ns;
+    }
+    for (int i = 0; i < ns; ++i) s->chk_host[i] = (int32_t)(splitmix64(s->check_seed) % (uint64_t)n);
+    HIP_TRY(s, hipEventRecord(s->chk_go, s->stream));
+    HIP_TRY(s, hipStreamWaitEvent(s->side, s->chk_go, 0));
+    HIP_TRY(s, hipMemcpyAsync(s->chk_idx, s->chk_host, (size_t)ns * sizeof(int32_t), hipMemcpyHostToDevice, s->side));
+    HIP_TRY(s, hipEventRecord(s->chk_ev[0], s->side));
+    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, false, s->side);
+    if (e == hipSuccess) e = hipEventRecord(s->chk_ev[1], s->side);
+    if (e != hipSuccess) return fail(s, NB_E_HIP, "sampled check launch: %s", hipGetErrorString(e));
+    s->chk_ns = ns;
+    return NB_OK;
+}
+
+// Force stage of a check step (its direct sum started with the step): the
+// learned forces are stored (velocity correction included), the check_samples
+// bodies (drawn with replacement) are compared with the FP64 direct sum, and
+// if the median relative error is above check_tol the step's forces are
+// recomputed with plain theta. The second pass corrects the velocities by
+// (a_plain - a_learned) dt / 2, so the step ends exactly as a plain step
+// would. The fallback holds until a new model is installed.
 int checked_force(nb_sim *s)
 {
     int r = launch_force_method(s, NB_MODE_LEARNED, 0);
     if (r) return r;
-    const int n = s->d.n;
-    const int ns = std::min(s->p.check_samples, n);
-    r = ensure_check_buffers(s, ns);
-    if (r) return r;
-    for (auto &ev : s->chk_ev)
-        if (!ev) HIP_TRY(s, hipEventCreate(&ev));
-    std::vector<int32_t> idx((size_t)ns);
-    for (int i = 0; i < ns; ++i) idx[i] = (int32_t)(splitmix64(s->check_seed) % (uint64_t)n);
-    HIP_TRY(s, hipMemcpyAsync(s->chk_idx, idx.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(s, hipEventRecord(s->chk_ev[0], s->stream));
-    hipError_t e = nb::launch_direct64(s->d, s->chk_idx, ns, s->chk_part, s->chk_out, false, s->stream);
-    if (e == hipSuccess) e = nb::launch_gather_acc(s->d, s->chk_idx, ns, s->chk_acc, s->stream);
-    if (e == hipSuccess) e = hipEventRecord(s->chk_ev[1], s->stream);
+    const int ns = s->chk_ns;
+    HIP_TRY(s, hipStreamWaitEvent(s->stream, s->chk_ev[1], 0));
+    hipError_t e = nb::launch_gather_acc(s->d, s->chk_idx, ns, s->chk_acc, s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "sampled check launch: %s", hipGetErrorString(e));
     std::vector<double4> ref;
     std::vector<float4> got;
     r = fetch(s, ref, (const double4 *)s->chk_out, (size_t)ns);
     if (!r) r = fetch(s, got, (const float4 *)s->chk_acc, (size_t)ns);
     if (r) return r;
-    float ms = 0.f; // both events have completed: the fetches drained the stream
+    float ms = 0.f; // both events have completed: the stream waited for the second
     HIP_TRY(s, hipEventElapsedTime(&ms, s->chk_ev[0], s->chk_ev[1]));
     s->check_ms += ms;
     ++s->checks;
@@ -709,6 +742,10 @@ int nb_step(nb_sim *s, int32_t nsteps)
         // back to plain theta if the median error is above check_tol
         const bool check = learned && !s->fell_back && s->p.check_every > 0 && s->p.check_tol > 0.0f &&
                            s->p.check_samples > 0 && (s->steps_since_model % s->p.check_every) == 0;
+        if (check) {
+            const int r = start_check(s);
+            if (r) return r;
+        }
         for (int st = NB_STAGE_BBOX; st <= NB_STAGE_INTEGRATE; ++st) {
             if (ev) HIP_TRY(s, hipEventRecord(ev[st - 1], s->stream));
             int r;
