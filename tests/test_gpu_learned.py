@@ -256,6 +256,33 @@ def test_check_fallback_to_plain(nb):
         s.close()
 
 
+def test_check_steps_in_one_call(nb):
+    """The check's direct sum runs on a second stream beside the step: six
+    steps in one call (three check steps, with the host queueing ahead) end
+    bit-identical to six single-step calls, and the counters report the
+    checks and their device time."""
+    from nbody_amd import universe as U
+
+    st = U.plummer(20000, seed=15)
+    blob = _random_model(4)
+    kw = dict(method=nb._lib.NB_MODE_LEARNED, coarse_alpha=ALPHA, check_tol=10.0, check_every=2,
+              check_samples=256, flags=nb._lib.NB_FUSED_UPDATE)
+    a = _coarse_sim(st, **kw)
+    b = _coarse_sim(st, **kw)
+    a.set_model(blob)
+    b.set_model(blob)
+    a.steps(6)
+    for _ in range(6):
+        b.steps(1)
+    for u, w in zip(a.state(), b.state()):
+        assert np.array_equal(u.view(np.uint32), w.view(np.uint32))
+    for s in (a, b):
+        c = s.counters()
+        assert c.checks == 3 and c.check_ms > 0.0 and c.fallbacks == 0
+        assert 0.0 < c.last_check_median < 10.0
+        s.close()
+
+
 @pytest.mark.slow
 def test_training_ablation(nb, tmp_path):
     """What the trained network gains over the coarse traversal alone, at the
