@@ -61,14 +61,15 @@ enum DevError : int {
 };
 
 // Device-resident scalars of one simulation (the reference keeps these in
-// one-element buffers: _step, _blockCount, _radius, _maxDepth, _bottom, _error).
+// one-element buffers: _step, _radius, _maxDepth, _bottom, _error; its
+// _blockCount, the bounding box's last-block counter, has no counterpart here:
+// a second launch finishes the box).
 struct Ctl {
     int step;            // reference _step, starts at -1 (java:167)
     uint32_t epoch;      // tree generation: tags child slots
     int n_cells;         // cell allocation counter (reference _bottom)
     int max_depth;       // running maximum (reference _maxDepth, java:170)
     int error;           // DevError
-    uint32_t unused0;
     float radius;        // reference _radius
     float root[3];
     float dq[kMaxDepth]; // opening thresholds (calculateforce.cl:52-67)
