@@ -188,6 +188,30 @@ def test_known_trees(nb):
     sim.close()
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_single_body(nb, oracle, fused):
+    """N = 1: the root holds the body, which meets only itself (d = 0, so no
+    force: calculateforce.cl:138-151); it drifts at constant velocity.
+    Three steps against the oracle bit for bit and against x0 + 3 v dt."""
+    from nbody_amd import _lib
+
+    st = tuple(np.array([v], np.float32) for v in (0.25, -0.5, 1.0, 0.3, -0.2, 0.1, 2.0))
+    sim = make_sim(st, flags=_lib.NB_FUSED_UPDATE if fused else 0, **PHYS)
+    orc = make_oracle(st, 0.5, 0.05, 1e-3, 16)
+    sim.steps(3)
+    assert orc.step(3) == 0
+    assert np.array_equal(sim.accel(), np.zeros((1, 3), np.float32))
+    g = np.array(sim.state(), np.float32)
+    o = np.array(orc.state(), np.float32)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (g, o)
+    x0 = np.array(st[:3], np.float64).ravel()
+    v0 = np.array(st[3:6], np.float64).ravel()
+    assert np.allclose(g[:3].ravel(), x0 + 3 * 1e-3 * v0, rtol=0, atol=1e-6)
+    assert np.array_equal(g[3:].ravel(), np.array(st[3:6], np.float32).ravel())
+    assert sim.counters().step == 2
+    sim.close()
+
+
 def test_coincident_bodies_report_error(nb):
     """Two bodies at identical FP32 coordinates never separate (the reference
     exhausts its cell pool, buildtree.cl:112-120): the step must fail with a
