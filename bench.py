@@ -48,7 +48,7 @@ ALPHA_CANDIDATES = (0.03, 0.025, 0.0225, 0.02, 0.0175, 0.015, 0.0125)
 CALIBRATION_TARGET = 0.9e-3
 ERROR_BAR = 1e-3
 TRAIN = dict(steps=50, groups_per_step=512, iters=4000, batch=4096, seed=1)
-CHECK = dict(check_tol=1e-3, check_every=10, check_samples=1024)
+CHECK = dict(check_tol=1e-3, check_every=10, check_samples=1024, check_adapt=1)
 SOFTENING = 0.01
 DT = 1e-3
 VOTE = 64

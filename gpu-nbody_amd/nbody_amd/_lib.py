@@ -54,7 +54,8 @@ class NbParams(C.Structure):
         ("check_every", C.c_int32),
         ("check_samples", C.c_int32),
         ("coarse_alpha", C.c_float),
-        ("reserved", C.c_int32 * 6),
+        ("check_adapt", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -72,7 +73,8 @@ class NbCounters(C.Structure):
         ("cell_interactions", C.c_int64),
         ("check_ms", C.c_double),
         ("checks", C.c_int64),
-        ("reserved", C.c_int64 * 2),
+        ("coarse_alpha", C.c_double),
+        ("reserved", C.c_int64 * 1),
     ]
 
 

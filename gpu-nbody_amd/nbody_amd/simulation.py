@@ -71,7 +71,7 @@ def make_params(nbodies: int, *, theta: Optional[float] = None, softening: Optio
                 device: int = 0, pool_factor: Optional[float] = None, method: int = _lib.NB_MODE_PLAIN,
                 coarse_theta: Optional[float] = None, check_tol: Optional[float] = None,
                 check_every: Optional[int] = None, check_samples: Optional[int] = None,
-                coarse_alpha: Optional[float] = None) -> NbParams:
+                coarse_alpha: Optional[float] = None, check_adapt: Optional[int] = None) -> NbParams:
     """nb_params with the reference's defaults (nb_default_params) and overrides."""
     lib = _lib.load()
     p = NbParams()
@@ -80,7 +80,7 @@ def make_params(nbodies: int, *, theta: Optional[float] = None, softening: Optio
                       ("vote_width", vote_width), ("pool_factor", pool_factor),
                       ("coarse_theta", coarse_theta), ("check_tol", check_tol),
                       ("check_every", check_every), ("check_samples", check_samples),
-                      ("coarse_alpha", coarse_alpha)):
+                      ("coarse_alpha", coarse_alpha), ("check_adapt", check_adapt)):
         if val is not None:
             setattr(p, name, val)
     p.flags = int(flags)

@@ -72,7 +72,12 @@ typedef struct nb_params {
     int32_t check_samples;/* learned mode: bodies in the sampled check */
     float coarse_alpha;   /* learned mode: relative test m b^3 / s^5 <= coarse_alpha |a_old|
                              of the coarse traversal (0 = off) */
-    int32_t reserved[6];
+    int32_t check_adapt;  /* learned mode, what a failed check does after its step fell back
+                             to plain theta: 0 = plain theta from then on (until a new
+                             model); 1 = tighten coarse_alpha for the following steps (aiming
+                             at 85 % of check_tol) and keep the learned force, with plain theta
+                             from then on after 3 failed checks in a row */
+    int32_t reserved[5];
 } nb_params;
 
 /* Defaults are the reference's shipped constants: THETA 1.5, EPSILON 0.0025,
@@ -166,7 +171,8 @@ typedef struct nb_counters {
     double check_ms;       /* learned mode: device time of the sampled FP64 checks (direct sum
                               and gather) since nb_create, cumulative */
     int64_t checks;        /* learned mode: sampled checks run since nb_create */
-    int64_t reserved[2];
+    double coarse_alpha;   /* learned mode: the relative test's alpha in effect (check_adapt) */
+    int64_t reserved[1];
 } nb_counters;
 int nb_get_counters(nb_sim *sim, nb_counters *out);
 

@@ -72,6 +72,7 @@ def test_create_rejects_bad_params_without_gpu(nb, field, value):
     dict(coarse_alpha=-0.01),                 # negative relative bound
     dict(coarse_alpha=float("inf")),
     dict(vote_width=16),                      # the coarse traversal is one wave64 per group
+    dict(check_adapt=2),                      # 0 or 1
 ])
 def test_learned_mode_parameter_checks(nb, kw):
     lib = nb.load()
@@ -85,6 +86,7 @@ def test_learned_mode_parameter_checks(nb, kw):
 def test_default_learned_parameters(nb):
     p = nb.make_params(1000)
     assert np.isclose(p.coarse_theta, 0.95) and np.isclose(p.coarse_alpha, 0.02)
+    assert p.check_adapt == 0  # a failed check keeps plain theta, as before the option existed
 
 
 def test_null_handles(nb):

@@ -15,7 +15,9 @@ Bars:
   ablation          after training on the simulation's own pairs, the learned
                     method's median error against the FP64 direct sum is below
                     the coarse traversal's alone on the same state
-  fallback          a check tolerance nobody meets makes the step plain theta
+  fallback          a check tolerance nobody meets makes the step plain theta;
+                    with check_adapt alpha tightens instead (exact halvings at
+                    such a tolerance) until three failures in a row
 """
 import numpy as np
 import pytest
@@ -280,6 +282,43 @@ def test_check_steps_in_one_call(nb):
         c = s.counters()
         assert c.checks == 3 and c.check_ms > 0.0 and c.fallbacks == 0
         assert 0.0 < c.last_check_median < 10.0
+        s.close()
+
+
+def test_check_adapt_tightens_alpha(nb):
+    """check_adapt: a failed check (tolerance 1e-12 fails every one) makes its
+    step plain theta and tightens alpha for the next steps, by at most a factor
+    2 each; after three failed checks in a row plain theta holds and checks
+    stop. Without check_adapt the first failed check ends the learned force."""
+    from nbody_amd import universe as U
+
+    st = U.plummer(20000, seed=16)
+    blob = _random_model(5)
+    kw = dict(method=nb._lib.NB_MODE_LEARNED, coarse_alpha=ALPHA, check_tol=1e-12, check_every=1,
+              check_samples=256)
+    a = _coarse_sim(st, check_adapt=1, **kw)
+    a.set_model(blob)
+    alphas = []
+    for _ in range(5):
+        a.steps(1)
+        c = a.counters()
+        alphas.append(c.coarse_alpha)
+    c = a.counters()
+    assert c.fallbacks == 3 and c.checks == 3, (c.fallbacks, c.checks)
+    assert np.isclose(alphas[0], ALPHA * 0.5, rtol=1e-6) and np.isclose(alphas[1], ALPHA * 0.25, rtol=1e-6)
+    assert alphas[2] == alphas[1] == alphas[4]  # the third failure ends the learned force instead
+    b = _coarse_sim(st, check_adapt=0, **kw)
+    b.set_model(blob)
+    b.steps(5)
+    c = b.counters()
+    assert c.fallbacks == 1 and c.checks == 1 and np.isclose(c.coarse_alpha, ALPHA)
+    # a loose tolerance leaves alpha alone
+    l = _coarse_sim(st, check_adapt=1, **dict(kw, check_tol=10.0))
+    l.set_model(blob)
+    l.steps(3)
+    c = l.counters()
+    assert c.fallbacks == 0 and c.checks == 3 and np.isclose(c.coarse_alpha, ALPHA)
+    for s in (a, b, l):
         s.close()
 
 
