@@ -19,7 +19,8 @@ on 4096 sampled bodies:
           theta = 0.4 residuals before the timed region; alpha is the largest
           of 0.03, 0.025, 0.0225, 0.02, 0.0175, 0.015, 0.0125 whose calibration
           error (a separate sample) is <= 0.9e-3. The sampled FP64 check (1024
-          bodies every 10 steps, tolerance 1e-3, plain theta as the fallback)
+          bodies every 10 steps, tolerance 1e-3, plain theta as the fallback
+          for a failing step, which also tightens alpha)
           runs inside the timed steps and counts in `value` (its direct sum runs
           on a second stream beside the step's other work); `config` also gives
           its device time and the steps/s of steps between checks.
