@@ -106,3 +106,30 @@ def test_c5_scale_run_learned(nb):
     assert np.isfinite(e1["energy"]) and drift < 1e-3
     assert err["median"] < 2e-3
     sim.close()
+
+
+def test_c4_thousand_steps_with_check_adapt(nb):
+    """The benchmark's workload (C4: 4,194,304 clustered bodies, coarse theta
+    0.99, alpha 0.0175, checked every 10 steps at tolerance 1e-3) over 1000
+    steps after training. The clusters are not in equilibrium (their kinetic
+    energy halves over these steps), so the calibration drifts: a failed check
+    tightens alpha, the learned force stays on (100 checks, a few fallback
+    steps), and the error at the end is still within the tolerance."""
+    from nbody_amd import universe as U
+
+    st = U.gaussian_clusters(4_194_304, seed=4)
+    sim = make_sim(st, theta=0.4, softening=0.01, dt=1e-3, vote_width=64, coarse_theta=0.99,
+                   coarse_alpha=0.0175, method=nb._lib.NB_MODE_LEARNED, check_tol=1e-3, check_every=10,
+                   check_samples=1024, check_adapt=1, flags=nb._lib.NB_FP64_POSITIONS | nb._lib.NB_FUSED_UPDATE)
+    sim.train(steps=50, groups_per_step=512, iters=4000, batch=4096, seed=1)
+    t0 = time.perf_counter()
+    sim.steps(1000)
+    dt = time.perf_counter() - t0
+    c = sim.counters()
+    err = sim.force_error(4096, seed=1000)
+    print(f"C4, 1000 steps: {1000 / dt:.1f} steps/s, alpha {c.coarse_alpha:.4f}, checks {c.checks}, "
+          f"fallbacks {c.fallbacks}, median error {err['median']:.3e}")
+    assert c.checks == 100  # the learned force held to the end
+    assert 1 <= c.fallbacks <= 4 and c.coarse_alpha < 0.0175
+    assert err["median"] < 1e-3
+    sim.close()
