@@ -44,7 +44,8 @@ struct nb_sim {
     bool hint_valid = false;
     int hint_levels = 0;
     int hint_start[nb::kMaxDepth + 2] = {};
-    int split = 0; // levels split.. go to the single-block tail launches
+    int split = 0; // levels split.. go to one single-block launch (the deepest levels)
+    int head = 0;  // levels ..head-1 go to another (the top levels)
     // learned mode
     nb::Learn L{};
     void *mlpbuf = nullptr;      // packed inference network (nb::kMlpBufBytes)
@@ -196,8 +197,9 @@ int hint_cells(const nb_sim *s, int level)
     return level < s->hint_levels ? s->hint_start[level + 1] - s->hint_start[level] : 0;
 }
 
-// Levels whose expected size is at most this many cells go to the tail
-// launch from the bottom up (two passes of its block each).
+// Levels whose expected size is at most this many cells (two passes of the
+// block each) go to the single-block launches: from the bottom up to the
+// first larger level, and from the top down to the first larger level.
 constexpr int kTailCells = 2 * nb::kTailThreads / 8;
 
 // Groups the live cells by level. Without level sizes of an earlier tree (the
@@ -215,7 +217,10 @@ int prepare_levels(nb_sim *s)
     }
     int split = s->hint_levels;
     while (split > 0 && hint_cells(s, split - 1) <= kTailCells) --split;
+    int head = 0;
+    while (head < split && hint_cells(s, head) <= kTailCells) ++head;
     s->split = split;
+    s->head = head;
     return NB_OK;
 }
 
@@ -263,15 +268,17 @@ int launch_stage(nb_sim *s, int stage)
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
-        e = nb::launch_summarize_tail(d, s->split, s->stream);
-        for (int L = s->split - 1; L >= 0 && e == hipSuccess; --L)
+        e = nb::launch_summarize_block(d, s->split, nb::kMaxDepth + 1, s->stream);
+        for (int L = s->split - 1; L >= s->head && e == hipSuccess; --L)
             e = nb::launch_summarize_level(d, L, hint_cells(s, L), s->stream);
+        if (e == hipSuccess && s->head > 0) e = nb::launch_summarize_block(d, 0, s->head, s->stream);
         break;
     }
     case NB_STAGE_SORT:
-        for (int L = 0; L < s->split && e == hipSuccess; ++L)
+        if (s->head > 0) e = nb::launch_sort_block(d, 0, s->head, s->stream);
+        for (int L = s->head; L < s->split && e == hipSuccess; ++L)
             e = nb::launch_sort_level(d, L, hint_cells(s, L), s->stream);
-        if (e == hipSuccess) e = nb::launch_sort_tail(d, s->split, s->stream);
+        if (e == hipSuccess) e = nb::launch_sort_block(d, s->split, nb::kMaxDepth + 1, s->stream);
         break;
     case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:

@@ -116,13 +116,13 @@ hipError_t launch_build(const Dev &d, int blocks, hipStream_t st);
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
 // Summarize and sort read each level's cell count on the device; the host's
 // cells_hint (the level's size in an earlier tree) only sizes the grid. The
-// tail launches take, in one block, every level from `split` to the tree's
-// last: summarize runs its tail before the levels split-1 .. 0, sort after
-// the levels 0 .. split-1.
+// block launches take levels lo .. hi-1 in one block (hi = kMaxDepth + 1:
+// every level from lo down, however deep the tree): the deepest levels and
+// the top ones, where the levels hold few cells.
 hipError_t launch_summarize_level(const Dev &d, int level, int cells_hint, hipStream_t st);
-hipError_t launch_summarize_tail(const Dev &d, int split, hipStream_t st);
+hipError_t launch_summarize_block(const Dev &d, int lo, int hi, hipStream_t st);
 hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st);
-hipError_t launch_sort_tail(const Dev &d, int split, hipStream_t st);
+hipError_t launch_sort_block(const Dev &d, int lo, int hi, hipStream_t st);
 // mode: 0 = store accelerations (reference semantics), 1 = also integrate
 // (fused kick-drift-kick), 2 = probe: write accelerations to d.probe only
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);

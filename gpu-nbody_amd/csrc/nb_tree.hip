@@ -656,16 +656,16 @@ __global__ __launch_bounds__(256) void k_summarize_level(Dev d, int level)
         summarize_cell<QUAD>(d, first, i0 + (threadIdx.x >> 3), k, cells, ep);
 }
 
-// The deepest levels, from the tree's last level up to `split`, in one block:
-// a barrier between levels makes each level's results visible to the next.
-// It takes every level below the launched ones, however deep the tree is.
+// Levels lo .. hi-1 (those the tree has), bottom-up, in one block: a barrier
+// between levels makes each level's results visible to the next. With hi past
+// the tree's depth it takes every level from lo down, however deep the tree.
 template <bool QUAD>
-__global__ __launch_bounds__(kTailThreads) void k_summarize_tail(Dev d, int split)
+__global__ __launch_bounds__(kTailThreads) void k_summarize_block(Dev d, int lo, int hi)
 {
     const Ctl *ctl = d.ctl;
     const uint32_t ep = ctl->epoch;
     const int k = threadIdx.x & 7;
-    for (int L = ctl->levels - 1; L >= split; --L) {
+    for (int L = min(hi, ctl->levels) - 1; L >= lo; --L) {
         const int first = ctl->lvl_start[L], cells = ctl->lvl_start[L + 1] - first;
         for (int i0 = 0; i0 < cells; i0 += kTailThreads / 8)
             summarize_cell<QUAD>(d, first, i0 + (threadIdx.x >> 3), k, cells, ep);
@@ -726,13 +726,14 @@ __global__ __launch_bounds__(256) void k_sort_level(Dev d, int level)
         sort_cell(d, level, first, i0 + (threadIdx.x >> 3), k, cells);
 }
 
-// Levels `split` .. the tree's last, top-down, in one block (as k_summarize_tail).
-__global__ __launch_bounds__(kTailThreads) void k_sort_tail(Dev d, int split)
+// Levels lo .. hi-1 (those the tree has), top-down, in one block (as
+// k_summarize_block).
+__global__ __launch_bounds__(kTailThreads) void k_sort_block(Dev d, int lo, int hi)
 {
     const Ctl *ctl = d.ctl;
-    const int levels = ctl->levels;
+    const int end = min(hi, ctl->levels);
     const int k = threadIdx.x & 7;
-    for (int L = split; L < levels; ++L) {
+    for (int L = lo; L < end; ++L) {
         const int first = ctl->lvl_start[L], cells = ctl->lvl_start[L + 1] - first;
         for (int i0 = 0; i0 < cells; i0 += kTailThreads / 8)
             sort_cell(d, L, first, i0 + (threadIdx.x >> 3), k, cells);
@@ -853,12 +854,12 @@ hipError_t launch_summarize_level(const Dev &d, int level, int cells_hint, hipSt
         hipLaunchKernelGGL(k_summarize_level<false>, level_grid(cells_hint), dim3(256), 0, st, d, level);
     return hipGetLastError();
 }
-hipError_t launch_summarize_tail(const Dev &d, int split, hipStream_t st)
+hipError_t launch_summarize_block(const Dev &d, int lo, int hi, hipStream_t st)
 {
     if (d.quad)
-        hipLaunchKernelGGL(k_summarize_tail<true>, dim3(1), dim3(kTailThreads), 0, st, d, split);
+        hipLaunchKernelGGL(k_summarize_block<true>, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
     else
-        hipLaunchKernelGGL(k_summarize_tail<false>, dim3(1), dim3(kTailThreads), 0, st, d, split);
+        hipLaunchKernelGGL(k_summarize_block<false>, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
     return hipGetLastError();
 }
 hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st)
@@ -866,9 +867,9 @@ hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_
     hipLaunchKernelGGL(k_sort_level, level_grid(cells_hint), dim3(256), 0, st, d, level);
     return hipGetLastError();
 }
-hipError_t launch_sort_tail(const Dev &d, int split, hipStream_t st)
+hipError_t launch_sort_block(const Dev &d, int lo, int hi, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sort_tail, dim3(1), dim3(kTailThreads), 0, st, d, split);
+    hipLaunchKernelGGL(k_sort_block, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
     return hipGetLastError();
 }
 hipError_t launch_integrate(const Dev &d, hipStream_t st)
