@@ -24,8 +24,9 @@ on 4096 sampled bodies:
           runs inside the timed steps and counts in `value` (its direct sum runs
           on a second stream beside the step's other work); `config` also gives
           its device time and the steps/s of steps between checks.
-`value` is the steps/s of the requested mode (learned by default); the other
-mode's steps/s, both errors and each stage's fraction of the HBM roof are in
+`value` is the steps/s of the requested mode (learned by default) over the
+K timed steps; the other mode's steps/s, both errors, each stage's fraction
+of the HBM roof and the median of three consecutive runs of K steps are in
 `config`.
 
 Prints one JSON line. One GPU only: --gpus N > 1 exits with a message.
@@ -322,12 +323,14 @@ def main():
         log(f"[{m}] timing {args.steps} steps after {args.warmup} warm-up steps")
         sps, ms, check_ms, checks = timed_run(sim, args.steps, args.warmup)
         fallbacks = sim.counters().fallbacks
+        # two more runs of K steps right after: the median of three runs
+        runs = [sps] + [timed_run(sim, args.steps, 0)[0] for _ in range(2)]
         sps_nc = timed_between_checks(sim) if m == "learned" else None
         stages, inter, cell_inter, live = stage_profile(sim, max(3, min(args.steps, 10)))
         err = sim.force_error(args.samples, seed=1000)
         results[m] = dict(steps_per_s=sps, ms_per_step=ms, stages=stages, interactions=inter,
                           cell_interactions=cell_inter, err=err, fallbacks=fallbacks, check_ms=check_ms,
-                          steps_per_s_between_checks=sps_nc,
+                          steps_per_s_between_checks=sps_nc, runs=runs,
                           checks=checks, live_cells=live,
                           rooflines=stage_rooflines(stages, args.n, live, not args.unfused, m == "learned"))
         if args.copy_calibration:
@@ -381,6 +384,9 @@ def main():
             "interactions_per_step": int(main_r["interactions"]),
             "cell_interactions_per_step": int(main_r["cell_interactions"]),
             "interactions_per_s": round(main_r["interactions"] / (force_ms * 1e-3), 1),
+            # `value` is the first run of K steps; two more runs follow it
+            "steps_per_s_runs": [round(r, 4) for r in main_r["runs"]],
+            "steps_per_s_median_of_3_runs": round(float(np.median(main_r["runs"])), 4),
         },
         "roofline": {
             "kernel": "force traversal (" + ("k_coarse64, coarse traversal" if mode == "learned" else "k_force64") + ")",
@@ -408,6 +414,8 @@ def main():
             "plain_steps_per_s": round(p["steps_per_s"], 4),
             "plain_ms_per_step": round(p["ms_per_step"], 3),
             "speedup_vs_plain": round(main_r["steps_per_s"] / p["steps_per_s"], 3),
+            "plain_steps_per_s_median_of_3_runs": round(float(np.median(p["runs"])), 4),
+            "speedup_vs_plain_median_of_3_runs": round(float(np.median(main_r["runs"]) / np.median(p["runs"])), 3),
             "plain_median_rel_force_err": round(p["err"]["median"], 8),
             "plain_stage_ms": p["stages"],
             "both_within_1e-3": bool(main_r["err"]["median"] <= ERROR_BAR and p["err"]["median"] <= ERROR_BAR),
