@@ -243,13 +243,18 @@ int launch_force_method(nb_sim *s, int method, int fmode)
     switch (method) {
     case NB_MODE_PLAIN: e = nb::launch_force(d, s->p.vote_width, fmode, s->stream); break;
     case NB_MODE_COARSE:
-        e = nb::launch_coarse(d, s->L, nb::kOutFinish, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-                              fmode, s->stream);
+        // groups in descending cost of their last traversal (see launch_group_order)
+        e = nb::launch_group_order(d, s->stream);
+        if (e == hipSuccess)
+            e = nb::launch_coarse(d, s->L, nb::kOutFinish, nullptr, d.gorder, (d.n + 63) / 64, nullptr, nullptr,
+                                  nullptr, fmode, s->stream);
         break;
     case NB_MODE_LEARNED:
         if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
-        e = nb::launch_coarse(d, s->L, nb::kOutLearned, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-                              fmode, s->stream);
+        e = nb::launch_group_order(d, s->stream);
+        if (e == hipSuccess)
+            e = nb::launch_coarse(d, s->L, nb::kOutLearned, nullptr, d.gorder, (d.n + 63) / 64, nullptr, nullptr,
+                                  nullptr, fmode, s->stream);
         break;
     default: return fail(s, NB_E_ARG, "unknown force method %d", method);
     }
@@ -392,7 +397,7 @@ void free_all(nb_sim *s)
     nb::Dev &d = s->d;
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list, d.clvl,
-                    d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part,
+                    d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part, d.gcost, d.gorder,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
                     s->mlpbuf, s->chk_acc};
     for (void *p : ptrs)
@@ -658,6 +663,8 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = ok && dalloc(&d.bnode, (size_t)n) == hipSuccess && dalloc(&d.probe, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.sorted, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.bbox_part, (size_t)6 * nb::kBBoxBlocks) == hipSuccess;
+    const size_t ngroups = ((size_t)n + 63) / 64;
+    ok = ok && dalloc(&d.gcost, ngroups) == hipSuccess && dalloc(&d.gorder, ngroups) == hipSuccess;
     if (!ok) {
         fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
         return bail(NB_E_HIP);
@@ -667,6 +674,7 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int4), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.count, 0, cap * sizeof(int), s->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(d.gcost, 0, ngroups * sizeof(int), s->stream) == hipSuccess;
     nb::Ctl c = initial_ctl(s, 0);
     ok = ok && hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(s->stream) == hipSuccess;

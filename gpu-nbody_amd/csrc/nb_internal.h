@@ -50,6 +50,7 @@ constexpr int kBBoxBlocks = 1024;    // partial-result slots of the bounding box
 constexpr int kLevelBlocks = 512;    // blocks of the level histogram / scatter
 constexpr int kTailThreads = 512;    // the single block of the deepest summarize / sort levels
 constexpr unsigned char kNotLive = 0xff; // clvl of a cell that is not in the tree
+constexpr int kCostKeys = 128;       // cost classes of the coarse traversal's launch order
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 enum DevError : int {
@@ -108,6 +109,8 @@ struct Dev {
     int *sorted;
     float4 *probe;      // accelerations of a probe force pass (error checks)
     float *bbox_part; // 6 * kBBoxBlocks
+    int *gcost;       // per 64-body group: cost of its last coarse traversal (children tested)
+    int *gorder;      // the groups, most expensive first (the coarse traversal's launch order)
 };
 
 // ---- launchers (nb_tree.hip, nb_force.hip) ---------------------------------
@@ -195,6 +198,10 @@ hipError_t launch_make_target(const float4 *frame, const float4 *afine, int rows
 // x = (x - mean) / sd in place
 hipError_t launch_standardize(float *x, int rows, int cols, float *mu, float *sd, hipStream_t st);
 hipError_t launch_gather_acc(const Dev &d, const int *idx, int ns, float4 *out, hipStream_t st);
+// d.gorder = the (n + 63) / 64 groups by the cost of their last coarse
+// traversal (d.gcost), most expensive first, so that the longest waves start
+// first and the launch does not end on a few of them
+hipError_t launch_group_order(const Dev &d, hipStream_t st);
 
 struct Train {
     float *P, *G, *M, *V;  // parameters, gradient, Adam moments (kMlpParams each)

@@ -257,10 +257,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     float smb = 0.f; // mass of the bodies met (the accepted cells hold the rest)
     int nc = 0, nbod = 0;
     long long ninter = 0, ncell = 0;
+    int npop = 0;
     bool bad = false;
     while (sp > 0) {
         __builtin_amdgcn_wave_barrier();
         --sp;
+        ++npop;
         const unsigned e = __builtin_amdgcn_readfirstlane(stk[wv][sp]);
         const int c = (int)(e >> 6);
         const int dep = (int)(e & 63u);
@@ -365,6 +367,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         if (bad) break;
     }
     if (bad && lane == 0) flag_error(ctl, kErrStack);
+    if constexpr (OUT == kOutLearned || OUT == kOutFinish) {
+        // the group's cost for the next launch order, in interactions: a
+        // popped cell (its loads and up to eight tests) weighs about three
+        const long long cost = ninter + 3ll * npop;
+        if (lane == 0) d.gcost[g] = cost < 0x7fffffffll ? (int)cost : 0x7fffffff;
+    }
     const float ax = sx - kx, ay = sy - ky, az = sz - kz;
     const int slot = gslot * 64 + lane;
     if constexpr (OUT == kOutAccel) {
@@ -526,6 +534,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         __hip_atomic_fetch_add(&ctl->cell_interactions, (unsigned long long)ncell * nvalid, __ATOMIC_RELAXED,
                                AGENT);
     }
+}
+
+// One block: a histogram of the groups' cost classes (the class grows with
+// log2 of the cost, four classes per octave), then the groups by class, the
+// most expensive first (the order inside a class is not fixed; the forces do
+// not depend on the order in which groups run).
+__device__ __forceinline__ int cost_key(int c)
+{
+    if (c <= 0) return 0;
+    const int lg = 31 - __clz(c);
+    const int sub = lg >= 2 ? (c >> (lg - 2)) & 3 : 0;
+    return 1 + 4 * lg + sub; // <= 1 + 4 * 30 + 3 < kCostKeys
+}
+
+__global__ __launch_bounds__(1024) void k_group_order(const int *__restrict__ cost, int ng, int *__restrict__ order)
+{
+    __shared__ int h[kCostKeys];
+    if (threadIdx.x < kCostKeys) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) atomicAdd(&h[cost_key(cost[g])], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = kCostKeys - 1; k >= 0; --k) {
+            const int t = h[k];
+            h[k] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+    // the same costs as the histogram read (no kernel writes them meanwhile),
+    // so every group lands in [0, ng) exactly once
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) order[atomicAdd(&h[cost_key(cost[g])], 1)] = g;
+}
+
+hipError_t launch_group_order(const Dev &d, hipStream_t st)
+{
+    const int ng = (d.n + 63) / 64;
+    if (ng <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group_order, dim3(1), dim3(1024), 0, st, (const int *)d.gcost, ng, d.gorder);
+    return hipGetLastError();
 }
 
 hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqtab, const int *groups,
