@@ -66,6 +66,12 @@ struct nb_sim {
     hipEvent_t chk_go = nullptr; // the step's positions are final (recorded on stream)
     int32_t *chk_host = nullptr; // pinned sample indices, so the upload never blocks the host
     int chk_host_cap = 0, chk_ns = 0;
+    // the coarse traversal's group order for the next pass is worked out on a
+    // stream of its own (default priority) right after each pass, beside the
+    // next step's tree stages instead of in front of its traversal
+    hipStream_t ord = nullptr;
+    hipEvent_t ord_in = nullptr, ord_out = nullptr;
+    bool ord_ready = false; // d.gorder holds an order once ord_out has passed
     std::string err;
 };
 
@@ -234,6 +240,34 @@ const float *dq_plain_ptr(const nb_sim *s)
 // failed sampled check.
 int force_method(const nb_sim *s) { return s->fell_back ? NB_MODE_PLAIN : s->p.mode; }
 
+// The coarse and learned traversals take their 64-body groups from d.gorder,
+// most expensive first. The order for a pass was worked out on the order
+// stream after the previous pass (this stream waits for it; the first pass
+// works it out here), and the order for the next pass follows this one there:
+// it reads the costs this pass writes, and nothing else touches d.gcost or
+// d.gorder in between.
+int launch_coarse_ordered(nb_sim *s, int out, int fmode)
+{
+    const nb::Dev &d = s->d;
+    if (!s->ord) {
+        HIP_TRY(s, hipStreamCreateWithFlags(&s->ord, hipStreamNonBlocking));
+        HIP_TRY(s, hipEventCreateWithFlags(&s->ord_in, hipEventDisableTiming));
+        HIP_TRY(s, hipEventCreateWithFlags(&s->ord_out, hipEventDisableTiming));
+    }
+    if (s->ord_ready)
+        HIP_TRY(s, hipStreamWaitEvent(s->stream, s->ord_out, 0));
+    else
+        HIP_TRY(s, nb::launch_group_order(d, s->stream));
+    HIP_TRY(s, nb::launch_coarse(d, s->L, out, nullptr, d.gorder, (d.n + 63) / 64, nullptr, nullptr, nullptr, fmode,
+                                 s->stream));
+    HIP_TRY(s, hipEventRecord(s->ord_in, s->stream));
+    HIP_TRY(s, hipStreamWaitEvent(s->ord, s->ord_in, 0));
+    HIP_TRY(s, nb::launch_group_order(d, s->ord));
+    HIP_TRY(s, hipEventRecord(s->ord_out, s->ord));
+    s->ord_ready = true;
+    return NB_OK;
+}
+
 // Force stage with an explicit method; fmode as nb::launch_force (0 store,
 // 1 fused update, 2 probe).
 int launch_force_method(nb_sim *s, int method, int fmode)
@@ -242,20 +276,10 @@ int launch_force_method(nb_sim *s, int method, int fmode)
     hipError_t e = hipSuccess;
     switch (method) {
     case NB_MODE_PLAIN: e = nb::launch_force(d, s->p.vote_width, fmode, s->stream); break;
-    case NB_MODE_COARSE:
-        // groups in descending cost of their last traversal (see launch_group_order)
-        e = nb::launch_group_order(d, s->stream);
-        if (e == hipSuccess)
-            e = nb::launch_coarse(d, s->L, nb::kOutFinish, nullptr, d.gorder, (d.n + 63) / 64, nullptr, nullptr,
-                                  nullptr, fmode, s->stream);
-        break;
+    case NB_MODE_COARSE: return launch_coarse_ordered(s, nb::kOutFinish, fmode);
     case NB_MODE_LEARNED:
         if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
-        e = nb::launch_group_order(d, s->stream);
-        if (e == hipSuccess)
-            e = nb::launch_coarse(d, s->L, nb::kOutLearned, nullptr, d.gorder, (d.n + 63) / 64, nullptr, nullptr,
-                                  nullptr, fmode, s->stream);
-        break;
+        return launch_coarse_ordered(s, nb::kOutLearned, fmode);
     default: return fail(s, NB_E_ARG, "unknown force method %d", method);
     }
     if (e != hipSuccess) return fail(s, NB_E_HIP, "force launch: %s", hipGetErrorString(e));
@@ -395,6 +419,7 @@ int install_model(nb_sim *s, const float *blob)
 void free_all(nb_sim *s)
 {
     nb::Dev &d = s->d;
+    if (s->ord) (void)hipStreamSynchronize(s->ord); // it may still be writing d.gorder
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list, d.clvl,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part, d.gcost, d.gorder,
@@ -409,6 +434,9 @@ void free_all(nb_sim *s)
     if (s->chk_go) (void)hipEventDestroy(s->chk_go);
     if (s->chk_host) (void)hipHostFree(s->chk_host);
     if (s->side) (void)hipStreamDestroy(s->side);
+    if (s->ord_in) (void)hipEventDestroy(s->ord_in);
+    if (s->ord_out) (void)hipEventDestroy(s->ord_out);
+    if (s->ord) (void)hipStreamDestroy(s->ord);
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 

@@ -549,14 +549,53 @@ __device__ __forceinline__ int cost_key(int c)
     return c <= 0 ? 0 : 32 - __clz(c); // 1 + floor(log2 c) <= 31 < kCostKeys
 }
 
+// Claims h[key] + (rank of this lane among the wave's active lanes with the
+// same key) with one LDS atomic per distinct key in the wave (neighbouring
+// groups mostly share a class, so that is one or two atomics, not 64). Every
+// lane of the wave calls it; the loop runs once per distinct key.
+__device__ __forceinline__ int wave_claim(int *h, int key, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long rem = __ballot(active);
+    int slot = 0;
+    while (rem) {
+        const int leader = __ffsll((long long)rem) - 1;
+        const int k = __shfl(key, leader);
+        const bool mine = active && key == k;
+        const unsigned long long m = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&h[k], __popcll(m));
+        base = __shfl(base, leader);
+        if (mine) slot = base + __popcll(m & below);
+        rem &= ~m;
+    }
+    return slot;
+}
+
+// Each pass reads the costs kOrderBatch per thread at a time, all loads in
+// flight together (one block cannot hide a load's latency otherwise: the
+// first version, one load per round trip, took 103 us at 4M bodies).
+constexpr int kOrderBatch = 16;
+
 __global__ __launch_bounds__(1024) void k_group_order(const int *__restrict__ cost, int ng, int *__restrict__ order)
 {
     __shared__ int h[kCostKeys];
-    if (threadIdx.x < kCostKeys) h[threadIdx.x] = 0;
+    const int tid = threadIdx.x;
+    if (tid < kCostKeys) h[tid] = 0;
     __syncthreads();
-    for (int g = threadIdx.x; g < ng; g += blockDim.x) atomicAdd(&h[cost_key(cost[g])], 1);
+    for (int base = 0; base < ng; base += 1024 * kOrderBatch) { // uniform bound: whole waves call wave_claim
+        int key[kOrderBatch];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int g = base + j * 1024 + tid;
+            key[j] = g < ng ? cost_key(cost[g]) : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) (void)wave_claim(h, key[j], key[j] >= 0);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         int acc = 0;
         for (int k = kCostKeys - 1; k >= 0; --k) {
             const int t = h[k];
@@ -567,7 +606,19 @@ __global__ __launch_bounds__(1024) void k_group_order(const int *__restrict__ co
     __syncthreads();
     // the same costs as the histogram read (no kernel writes them meanwhile),
     // so every group lands in [0, ng) exactly once
-    for (int g = threadIdx.x; g < ng; g += blockDim.x) order[atomicAdd(&h[cost_key(cost[g])], 1)] = g;
+    for (int base = 0; base < ng; base += 1024 * kOrderBatch) {
+        int key[kOrderBatch];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int g = base + j * 1024 + tid;
+            key[j] = g < ng ? cost_key(cost[g]) : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int slot = wave_claim(h, key[j], key[j] >= 0);
+            if (key[j] >= 0) order[slot] = base + j * 1024 + tid;
+        }
+    }
 }
 
 hipError_t launch_group_order(const Dev &d, hipStream_t st)
