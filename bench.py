@@ -417,6 +417,7 @@ def main():
             "plain_steps_per_s_median_of_3_runs": round(float(np.median(p["runs"])), 4),
             "speedup_vs_plain_median_of_3_runs": round(float(np.median(main_r["runs"]) / np.median(p["runs"])), 3),
             "plain_median_rel_force_err": round(p["err"]["median"], 8),
+            "plain_p90_rel_force_err": round(p["err"]["p90"], 8),
             "plain_stage_ms": p["stages"],
             "both_within_1e-3": bool(main_r["err"]["median"] <= ERROR_BAR and p["err"]["median"] <= ERROR_BAR),
             "learned_calibration_median": round(learned_info["calibration_median"], 8),
