@@ -604,8 +604,9 @@ __global__ __launch_bounds__(1024) void k_group_order(const int *__restrict__ co
         }
     }
     __syncthreads();
-    // the same costs as the histogram read (no kernel writes them meanwhile),
-    // so every group lands in [0, ng) exactly once
+    // the same costs as the histogram read (the engine runs this kernel after
+    // the pass that wrote them and holds the next pass until it is done), so
+    // every group lands in [0, ng) exactly once
     for (int base = 0; base < ng; base += 1024 * kOrderBatch) {
         int key[kOrderBatch];
 #pragma unroll
