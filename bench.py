@@ -27,7 +27,9 @@ on 4096 sampled bodies:
 `value` is the steps/s of the requested mode (learned by default) over the
 K timed steps; the other mode's steps/s, both errors, each stage's fraction
 of the HBM roof and the median of three consecutive runs of K steps are in
-`config`.
+`config`. Also in `config`: the coarse traversal alone (no network) at its
+own calibrated alpha, timed the same way (what the network gains at equal
+error), and 1000 sustained learned steps after the timed ones.
 
 Prints one JSON line. One GPU only: --gpus N > 1 exits with a message.
 """
@@ -61,6 +63,8 @@ FLOP_PER_INTERACTION = 20  # GPU Gems 3 ch. 31 convention (Nyland, Harris, Prins
 FLOP_PER_CELL_INTERACTION = 56
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector packed = matrix dense) peak
 PEAK_HBM_GBS = 8000.0
+SUSTAINED_STEPS = 1000
+CPU_VOTE = 16  # the CPU baseline runs the reference's own work-group vote
 
 
 def log(msg):
@@ -75,6 +79,8 @@ def make_sim(state, mode, alpha=0.02, fused=True):
               flags=_lib.NB_FP64_POSITIONS | (_lib.NB_FUSED_UPDATE if fused else 0))
     if mode == "learned":
         kw.update(method=_lib.NB_MODE_LEARNED, coarse_theta=THETA_COARSE, coarse_alpha=alpha, **CHECK)
+    elif mode == "coarse":
+        kw.update(method=_lib.NB_MODE_COARSE, coarse_theta=THETA_COARSE, coarse_alpha=alpha)
     sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, len(state[0]), ArrayUniverse(state), **kw)
     sim.init(None)
     return sim
@@ -95,6 +101,24 @@ def train_learned(state, candidates, train):
         if chosen is not None:
             chosen[0].close()
         chosen = (sim, al, rep, cal)
+        if cal["median"] <= CALIBRATION_TARGET:
+            break
+    return chosen
+
+
+def calibrate_coarse(state, candidates):
+    """The coarse traversal alone (no network, no check): after the same 50
+    steps of simulated time as the learned run's training, the coarsest alpha
+    whose calibration error meets the same target."""
+    chosen = None
+    for al in candidates:
+        sim = make_sim(state, "coarse", al)
+        sim.steps(TRAIN["steps"])
+        cal = sim.force_error(4096, seed=4242)
+        log(f"[coarse] theta_c {THETA_COARSE} alpha {al}: calibration median {cal['median']:.3e}")
+        if chosen is not None:
+            chosen[0].close()
+        chosen = (sim, al, cal)
         if cal["median"] <= CALIBRATION_TARGET:
             break
     return chosen
@@ -156,12 +180,27 @@ def stage_rooflines(stages, n, cells, fused, learned):
     return out
 
 
-def stage_profile(sim, steps):
-    """Per-stage device time (HIP events on the engine's stream) over `steps`."""
+def stage_profile(sim, steps, checked=False):
+    """Per-stage device time (HIP events on the engine's stream) over `steps`.
+    With the learned mode's sampled check on (checked), the profiled steps are
+    ones without a check: single steps until one runs the check, then at most
+    check_every - 1 steps, and the check counter must not move in them (a
+    check step's direct sum on the second stream would share the GPU with
+    its tree stages)."""
+    if checked:
+        c0 = sim.counters().checks
+        for _ in range(CHECK["check_every"]):
+            sim.steps(1)
+            if sim.counters().checks != c0:
+                break
+        steps = min(steps, CHECK["check_every"] - 1)
+    c_before = sim.counters().checks
     sim.set_timing(True)
     sim.steps(steps)
     sim.set_timing(False)
     c = sim.counters()
+    if checked and c.checks != c_before:
+        raise RuntimeError("a sampled check ran inside the stage profile")
     names = ("bbox", "build", "summarize", "sort", "force", "integrate")
     return ({k: round(c.stage_ms[i], 4) for i, k in enumerate(names)}, c.interactions / steps,
             c.cell_interactions / steps, int(c.live_cells))
@@ -175,11 +214,11 @@ def _oracle():
     return pyoracle
 
 
-def _oracle_sampled_step(state, theta, softening, vote, flags, every=4):
-    """One oracle step: full bbox, build, summarize, sort and integrate, the
-    force on one vote group in `every`, scaled to all groups."""
-    import numpy as np
-
+def _cpu_step(state, theta, softening, vote, flags):
+    """One step of the reference algorithm on the host: the oracle's bounding
+    box, build, summarize and sort (sequential C), the FP32 force pass of
+    oracle/cpu_fp32.c over every vote group on the OpenMP threads, and the
+    oracle's integrate. Returns (tree s, force s, integrate s, interactions)."""
     pyoracle = _oracle()
     n = len(state[0])
     o = pyoracle.Oracle(n, theta=theta, softening=softening, dt=DT, vote=vote, flags=flags)
@@ -187,60 +226,87 @@ def _oracle_sampled_step(state, theta, softening, vote, flags, every=4):
     t0 = time.perf_counter()
     assert o.run_stages(1, 4) == 0
     t_tree = time.perf_counter() - t0
-    groups = (n + vote - 1) // vote
-    sample = np.arange(0, groups, every, dtype=np.int32)
     t0 = time.perf_counter()
-    assert o.force_groups(sample) == 0
-    t_force = (time.perf_counter() - t0) * groups / len(sample)
+    _, ni = o.force_fp32()
+    t_force = time.perf_counter() - t0
     t0 = time.perf_counter()
     o.run_stages(6, 6)
     t_int = time.perf_counter() - t0
     o.close()
-    return t_tree, t_force, t_int, len(sample), groups
+    return t_tree, t_force, t_int, ni
 
 
-def cpu_baseline(state, threads):
-    """The oracle (plain-C restatement of the reference step, OpenMP force) on
-    the host cores: a full tree build of the workload plus the force pass on
-    one vote group in four, scaled to a whole step."""
+def host_cpu():
+    """Threads the baseline uses and what the host reports: the GPU box gives
+    a process 16 cores per GPU, whatever os.cpu_count() shows."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model, avx512 = "unknown", False
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("flags") and " avx512f" in line:
+                    avx512 = True
+    except OSError:
+        pass
+    return {"threads": min(16, avail), "os_cpu_count": os.cpu_count(), "affinity_cpus": avail,
+            "model": model, "avx512f": avx512}
+
+
+def cpu_baseline(state, host):
+    """The reference step restated on the host cores (C, FP32 force, the
+    reference's vote width of 16) on the benchmark's workload: one full step,
+    every vote group."""
     pyoracle = _oracle()
+    threads = host["threads"]
     pyoracle.lib().orc_set_threads(threads)
     n = len(state[0])
-    t_tree, t_force, t_int, ns, groups = _oracle_sampled_step(state, THETA_PLAIN, SOFTENING, VOTE,
-                                                              pyoracle.FP64_POS)
+    t_tree, t_force, t_int, ni = _cpu_step(state, THETA_PLAIN, SOFTENING, CPU_VOTE, pyoracle.FP64_POS)
     step_s = t_tree + t_force + t_int
     return {"value": round(1.0 / step_s, 6), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle/bh_oracle.c (reference step restated in C, OpenMP force), plain theta "
-                       f"{THETA_PLAIN}, vote {VOTE}, N={n}: full bbox+build+summarize+sort "
-                       f"({t_tree:.2f} s) and integrate ({t_int:.2f} s), force on 1 of every 4 vote groups "
-                       f"({ns} groups) scaled x{groups / ns:.0f} to {t_force:.1f} s")}
+            "sample": (f"one full step of the workload (N={n}, plain theta {THETA_PLAIN}, vote {CPU_VOTE}): "
+                       f"bbox+build+summarize+sort of the oracle, sequential C ({t_tree:.2f} s); force on "
+                       f"every vote group in FP32 (oracle/cpu_fp32.c, calculateforce.cl:143-151 arithmetic, "
+                       f"{threads} OpenMP threads, {ni} interactions, {t_force:.2f} s); integrate "
+                       f"({t_int:.2f} s)"),
+            "cpu": host, "force_interactions_per_s": round(ni / t_force, 1)}
 
 
-def config_table(threads):
+def config_table(host):
     """The other configurations of the benchmark description with plain
-    Barnes-Hut, each timed on the GPU (HIP engine) and on the host cores (the
-    oracle): C1 in full (10 steps), one step of C2, and one step of C3 at
-    theta 0.4 with the force on one vote group in four, scaled."""
+    Barnes-Hut, each timed on the GPU (HIP engine) and one step on the host
+    cores (the oracle's tree stages and the FP32 force pass of every vote
+    group): C1, C2, and C3 at theta 0.3, 0.4 and 0.6 (the host at 0.4)."""
     import nbody_amd
     from nbody_amd import HipBarnesHutNBodySimulation, Mode, _lib
     from nbody_amd import universe as U
     from nbody_amd.universe import ArrayUniverse
 
     pyoracle = _oracle()
-    pyoracle.lib().orc_set_threads(threads)
+    pyoracle.lib().orc_set_threads(host["threads"])
     nbody_amd.load()
+    c3 = dict(softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE | _lib.NB_FP64_POSITIONS)
     cfgs = [
         ("C1", "uniform ball in rotation, N=16384", lambda: U.uniform_ball(16384, seed=1),
-         dict(theta=0.5, softening=0.05, vote_width=16, flags=0), 10, 1),
+         dict(theta=0.5, softening=0.05, vote_width=16, flags=0), 10, 1, True),
         ("C2", "Plummer sphere, N=1048576", lambda: U.plummer(1 << 20, seed=2),
-         dict(theta=0.5, softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE), 20, 3),
+         dict(theta=0.5, softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE), 20, 3, True),
         ("C3", "two Plummer spheres, N=4194304, FP64 positions", lambda: U.two_plummer(1 << 22, seed=3),
-         dict(theta=0.4, softening=0.01, vote_width=64, flags=_lib.NB_FUSED_UPDATE | _lib.NB_FP64_POSITIONS),
-         20, 3),
+         dict(theta=0.4, **c3), 20, 3, True),
+        ("C3_theta0.3", "two Plummer spheres, N=4194304, FP64 positions", None, dict(theta=0.3, **c3), 20, 3,
+         False),
+        ("C3_theta0.6", "two Plummer spheres, N=4194304, FP64 positions", None, dict(theta=0.6, **c3), 20, 3,
+         False),
     ]
     out = {}
-    for name, desc, gen, kw, steps, warm in cfgs:
-        st = gen()
+    st = None
+    for name, desc, gen, kw, steps, warm, on_cpu in cfgs:
+        if gen is not None:
+            st = gen()
         n = len(st[0])
         sim = HipBarnesHutNBodySimulation(Mode.DEFAULT, n, ArrayUniverse(st), dt=DT, **kw)
         sim.init(None)
@@ -248,28 +314,24 @@ def config_table(threads):
         t0 = time.perf_counter()
         sim.steps(steps)
         gpu_s = (time.perf_counter() - t0) / steps
+        err = sim.force_error(4096, seed=11) if name.startswith("C3") else None
         sim.close()
-        oflags = pyoracle.FP64_POS if kw["flags"] & _lib.NB_FP64_POSITIONS else 0
-        if name == "C1":
-            o = pyoracle.Oracle(n, theta=kw["theta"], softening=kw["softening"], dt=DT,
-                                vote=kw["vote_width"], flags=oflags)
-            o.set_state(*st)
-            t0 = time.perf_counter()
-            assert o.step(steps) == 0
-            cpu_s = (time.perf_counter() - t0) / steps
-            o.close()
-            how = f"{steps} full steps"
-        else:
-            every = 1 if name == "C2" else 4
-            t_tree, t_force, t_int, ns, groups = _oracle_sampled_step(st, kw["theta"], kw["softening"],
-                                                                      kw["vote_width"], oflags, every)
+        row = {"workload": desc, "theta": kw["theta"], "softening": kw["softening"],
+               "vote_width": kw["vote_width"], "gpu_ms_per_step": round(gpu_s * 1e3, 3),
+               "gpu_steps_per_s": round(1.0 / gpu_s, 2), "gpu_steps_timed": steps}
+        if err is not None:
+            row["median_rel_force_err"] = round(err["median"], 8)
+        if on_cpu:
+            oflags = pyoracle.FP64_POS if kw["flags"] & _lib.NB_FP64_POSITIONS else 0
+            t_tree, t_force, t_int, _ = _cpu_step(st, kw["theta"], kw["softening"], CPU_VOTE, oflags)
             cpu_s = t_tree + t_force + t_int
-            how = "1 full step" if every == 1 else f"1 step, force on {ns} of {groups} vote groups scaled"
-        out[name] = {"workload": desc, "theta": kw["theta"], "softening": kw["softening"],
-                     "vote_width": kw["vote_width"], "gpu_ms_per_step": round(gpu_s * 1e3, 3),
-                     "gpu_steps_timed": steps, "cpu_s_per_step": round(cpu_s, 4), "cpu_sample": how,
-                     "gpu_over_cpu": round(cpu_s / gpu_s, 1)}
-        log(f"[{name}] GPU {gpu_s * 1e3:.3f} ms/step, CPU {cpu_s:.3f} s/step ({how})")
+            row.update({"cpu_s_per_step": round(cpu_s, 4),
+                        "cpu_sample": f"1 full step, vote {CPU_VOTE}, FP32 force on {host['threads']} threads",
+                        "gpu_over_cpu": round(cpu_s / gpu_s, 1)})
+            log(f"[{name}] GPU {gpu_s * 1e3:.3f} ms/step, CPU {cpu_s:.3f} s/step")
+        else:
+            log(f"[{name}] GPU {gpu_s * 1e3:.3f} ms/step")
+        out[name] = row
     return out
 
 
@@ -289,6 +351,10 @@ def main():
     ap.add_argument("--train-iters", type=int, default=None, help="Adam iterations (default 4000)")
     ap.add_argument("--unfused", action="store_true",
                     help="plain mode with the integrate stage as its own kernel (for profiling it)")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the coarse-only run at its own calibrated alpha")
+    ap.add_argument("--no-sustained", action="store_true",
+                    help=f"skip the {SUSTAINED_STEPS} sustained learned steps after the timed ones")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run the vertex copy kernel once per mode (a copy of known size for HBM counters)")
     args = ap.parse_args()
@@ -326,7 +392,7 @@ def main():
         # two more runs of K steps right after: the median of three runs
         runs = [sps] + [timed_run(sim, args.steps, 0)[0] for _ in range(2)]
         sps_nc = timed_between_checks(sim) if m == "learned" else None
-        stages, inter, cell_inter, live = stage_profile(sim, max(3, min(args.steps, 10)))
+        stages, inter, cell_inter, live = stage_profile(sim, max(3, min(args.steps, 10)), checked=m == "learned")
         err = sim.force_error(args.samples, seed=1000)
         results[m] = dict(steps_per_s=sps, ms_per_step=ms, stages=stages, interactions=inter,
                           cell_interactions=cell_inter, err=err, fallbacks=fallbacks, check_ms=check_ms,
@@ -339,20 +405,53 @@ def main():
             vel4 = np.zeros((args.n, 4), np.float32)
             sim.copy_vertices(pos4, vel4)
         log(f"[{m}] {sps:.3f} steps/s, {ms:.2f} ms/step, median err {err['median']:.2e}, stages {stages}")
+        if m == "learned" and not args.no_sustained:
+            # the learned run carries on: SUSTAINED_STEPS more steps in one call,
+            # checks and check_adapt live, then the error at the end
+            c0 = sim.counters()
+            t0 = time.perf_counter()
+            sim.steps(SUSTAINED_STEPS)
+            t1 = time.perf_counter()
+            c1 = sim.counters()
+            e_end = sim.force_error(args.samples, seed=1001)
+            results[m]["sustained"] = {
+                "steps": SUSTAINED_STEPS, "steps_per_s": round(SUSTAINED_STEPS / (t1 - t0), 4),
+                "checks": int(c1.checks - c0.checks), "fallbacks": int(c1.fallbacks - c0.fallbacks),
+                "alpha_start": round(c0.coarse_alpha, 6), "alpha_end": round(c1.coarse_alpha, 6),
+                "median_rel_force_err_end": round(e_end["median"], 8)}
+            log(f"[{m}] sustained {results[m]['sustained']}")
         sim.close()
+    if mode == "learned" and not args.no_ablation:
+        # the coarse traversal alone at its own calibrated alpha: what the
+        # network buys at the same error target
+        csim, cal_alpha, ccal = calibrate_coarse(state, ALPHA_CANDIDATES)
+        csps, cms, _, _ = timed_run(csim, args.steps, args.warmup)
+        cruns = [csps] + [timed_run(csim, args.steps, 0)[0] for _ in range(2)]
+        cerr = csim.force_error(args.samples, seed=1000)
+        results["coarse"] = dict(steps_per_s=csps, ms_per_step=cms, runs=cruns, err=cerr, alpha=cal_alpha,
+                                 calibration_median=ccal["median"])
+        log(f"[coarse] alpha {cal_alpha}: {csps:.3f} steps/s, median err {cerr['median']:.2e}")
+        csim.close()
 
     main_r = results[mode]
     force_ms = main_r["stages"]["force"]  # the traversal (and, learned, the network and the body update)
     cells = main_r["cell_interactions"]
-    flop = FLOP_PER_INTERACTION * (main_r["interactions"] - cells) + FLOP_PER_CELL_INTERACTION * cells
-    achieved = flop / (force_ms * 1e-3) / 1e12
-    # HBM bytes per launch of the dominant kernel, from the rocprofv3 counter runs
-    # summarized in profiles/ (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated)
-    traffic = None
+    inter = main_r["interactions"]
+    # the convention of the scope table: 20 FLOP per interaction, cell or body
+    achieved = FLOP_PER_INTERACTION * inter / (force_ms * 1e-3) / 1e12
+    # beside it: cells of the coarse traversal priced with their quadrupole (56)
+    achieved_q = (FLOP_PER_INTERACTION * (inter - cells) + FLOP_PER_CELL_INTERACTION * cells) / (force_ms * 1e-3) / 1e12
+    # HBM bytes per launch of the dominant kernel: the rocprofv3 counter runs
+    # (FETCH_SIZE, WRITE_SIZE, copy-kernel calibrated) summarized in
+    # profiles/force_traffic.json, with the command and kernel they came from
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "force_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch", {}).get("k_coarse64" if mode == "learned" else "k_force64")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch", {}).get("k_coarse64" if mode == "learned" else "k_force64")
+        traffic_src = {"file": "profiles/force_traffic.json", "measured_on": tj.get("measured_on"),
+                       "command": tj.get("command")}
 
     line = {
         "metric": METRIC,
@@ -390,22 +489,29 @@ def main():
         },
         "roofline": {
             "kernel": "force traversal (" + ("k_coarse64, coarse traversal" if mode == "learned" else "k_force64") + ")",
-            "bound": "mfma",
-            "engine": "FP32 VALU (no matrix instructions); peak is MI355X FP32 157.3 TFLOP/s",
+            "bound": "valu",
+            "engine": ("FP32 vector ALU; peak is MI355X FP32 157.3 TFLOP/s" +
+                       ("; the BF16 matrix-core network in the kernel's epilogue is neither counted nor "
+                        "bounded here" if mode == "learned" else "")),
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "flop_per_interaction": FLOP_PER_INTERACTION,
-            "flop_per_cell_interaction": FLOP_PER_CELL_INTERACTION if mode == "learned" else None,
-            "flop_note": ("20 per body-body interaction (monopole convention), 56 per cell interaction of "
-                          "the coarse traversal (monopole + softened quadrupole); the network and the "
-                          "features are not counted") if mode == "learned" else "20 per interaction",
+            "interactions_per_launch": int(inter),
             "launch_ms": force_ms,
+            "flop_note": "20 FLOP per interaction (cell or body), the GPU Gems 3 ch. 31 convention",
         },
     }
     if mode == "learned":
+        line["roofline"].update({
+            "achieved_quadrupole_priced": round(achieved_q, 3),
+            "frac_quadrupole_priced": round(achieved_q / PEAK_FP32_TFLOPS, 4),
+            "quadrupole_note": ("cells of the coarse traversal priced at 56 FLOP (monopole 20 + softened "
+                                "quadrupole 36), bodies at 20"),
+            "cell_interactions_per_launch": int(cells)})
         p = results["plain"]
         tr = learned_info["train"]
         line["config"].update({
@@ -431,17 +537,30 @@ def main():
                                            if main_r["steps_per_s_between_checks"] else None),
             "speedup_vs_plain_between_checks": (round(main_r["steps_per_s_between_checks"] / p["steps_per_s"], 3)
                                                 if main_r["steps_per_s_between_checks"] else None),
+            "stage_ms_note": "HIP-event stage times over steps without a sampled check",
             "training": {"steps": TRAIN["steps"], "pairs": tr["samples"], "iters": TRAIN["iters"],
                          "batch": TRAIN["batch"], "pair_generation_ms": round(tr["gen_ms"], 1),
                          "adam_ms": round(tr["train_ms"], 1), "loss_first": round(tr["loss_first"], 4),
                          "loss_last": round(tr["loss_last"], 4)},
         })
+        if "sustained" in main_r:
+            line["config"]["sustained"] = main_r["sustained"]
+        if "coarse" in results:
+            c = results["coarse"]
+            line["config"]["ablation_coarse_only"] = {
+                "what": ("the coarse traversal alone (no network, no check) at its own calibrated alpha, "
+                         "same calibration target, timed the same way"),
+                "alpha": c["alpha"], "calibration_median": round(c["calibration_median"], 8),
+                "steps_per_s": round(c["steps_per_s"], 4),
+                "steps_per_s_median_of_3_runs": round(float(np.median(c["runs"])), 4),
+                "median_rel_force_err": round(c["err"]["median"], 8),
+                "learned_over_coarse_only": round(float(np.median(main_r["runs"]) / np.median(c["runs"])), 3)}
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(state, threads)
+        host = host_cpu()
+        log(f"cpu baseline on {host['threads']} threads ({host['model']})")
+        line["cpu_baseline"] = cpu_baseline(state, host)
         if not args.no_config_table:
-            line["cpu_baseline"]["configs"] = config_table(threads)
+            line["cpu_baseline"]["configs"] = config_table(host)
     print(json.dumps(line), flush=True)
 
 

@@ -59,9 +59,36 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+JAVA = os.path.join(HERE, "java")
+ROOT = os.path.dirname(HERE)
+SEQ_SRC = os.path.join(ROOT, "tests", "jni_sequence.c")
+SEQ_BIN = os.path.join(ROOT, "tests", "bin", "jni_sequence")
+
+
+def build_jni_sequence(verbose: bool = False) -> str:
+    """Compile the JNI glue's plain-C core (java/nbody_jni_core.c) with the
+    program that replays the Java class's call sequence through it
+    (tests/jni_sequence.c), linked against libnbody.so. The JNI wrappers
+    themselves (java/nbody_jni.c) need a JDK's jni.h, which this image lacks."""
+    os.makedirs(os.path.dirname(SEQ_BIN), exist_ok=True)
+    cc = shutil.which("gcc") or "cc"
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    rpath = "-Wl,-rpath,$ORIGIN/../../gpu-nbody_amd"
+    cmd = [cc, "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{JAVA}", SEQ_SRC,
+           os.path.join(JAVA, "nbody_jni_core.c"), f"-L{HERE}", "-lnbody", f"-L{rocm_lib}", rpath,
+           f"-Wl,-rpath,{rocm_lib}", "-lm", "-o", SEQ_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"jni sequence build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {SEQ_BIN}")
+    return SEQ_BIN
+
+
 if __name__ == "__main__":
     try:
         build(verbose=True)
+        build_jni_sequence(verbose=True)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -72,6 +72,9 @@ struct nb_sim {
     hipStream_t ord = nullptr;
     hipEvent_t ord_in = nullptr, ord_out = nullptr;
     bool ord_ready = false; // d.gorder holds an order once ord_out has passed
+    // copyVertices staging: 2 N float4 (positions, then velocities), allocated
+    // by the first nb_copy_vertices call and kept until nb_destroy
+    float4 *vtx = nullptr;
     std::string err;
 };
 
@@ -337,6 +340,37 @@ int ensure_learn(nb_sim *s)
     return ensure_quad(s);
 }
 
+// Waits for everything queued on the handle's streams (the main stream, the
+// sampled check's side stream, the launch-order stream); errors are ignored:
+// this runs before host data overwrites device buffers that queued kernels may
+// still read, and on the error returns of nb_step.
+void drain(nb_sim *s)
+{
+    for (hipStream_t st : {s->stream, s->side, s->ord})
+        if (st) (void)hipStreamSynchronize(st);
+}
+
+// Host -> device copy ordered on the handle's stream, complete on return.
+int upload(nb_sim *s, void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return NB_OK;
+}
+
+// Learned-mode control state back to what a new handle starts with: the
+// configured alpha, no fallback, the check schedule and its sample seed at
+// their start.
+void reset_control(nb_sim *s)
+{
+    s->alpha_now = s->p.coarse_alpha;
+    s->d.alpha2_coarse = s->p.coarse_alpha > 0.0f ? (float)((double)s->p.coarse_alpha * (double)s->p.coarse_alpha) : 0.0f;
+    s->fell_back = false;
+    s->fail_streak = 0;
+    s->steps_since_model = 0;
+    s->check_seed = 0x5eed;
+}
+
 uint16_t bf16_rne(float f)
 {
     uint32_t u;
@@ -396,7 +430,9 @@ int install_model(nb_sim *s, const float *blob)
         mu[i] = xmu[i];
         rs[i] = 1.0f / xsd[i];
     }
-    HIP_TRY(s, hipMemcpy(s->mlpbuf, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    drain(s);
+    r = upload(s, s->mlpbuf, buf.data(), buf.size());
+    if (r) return r;
     const unsigned char *base = static_cast<const unsigned char *>(s->mlpbuf);
     const size_t fo = (size_t)(reinterpret_cast<unsigned char *>(fb) - buf.data());
     s->L.mlp.w1f = reinterpret_cast<const uint16_t *>(base);
@@ -410,9 +446,7 @@ int install_model(nb_sim *s, const float *blob)
     s->L.mlp.xrs = dfb + 168;
     s->model.assign(blob, blob + NB_MODEL_FLOATS);
     s->have_model = true;
-    s->fell_back = false;
-    s->fail_streak = 0;
-    s->steps_since_model = 0;
+    reset_control(s);
     return NB_OK;
 }
 
@@ -424,7 +458,7 @@ void free_all(nb_sim *s)
                     d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list, d.clvl,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part, d.gcost, d.gorder,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
-                    s->mlpbuf, s->chk_acc};
+                    s->mlpbuf, s->chk_acc, s->vtx};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
@@ -729,33 +763,39 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     if (!s) return NB_E_ARG;
     if (!x || !y || !z || !vx || !vy || !vz || !m) return fail(s, NB_E_ARG, "null state array");
     HIP_TRY(s, hipSetDevice(s->device));
+    // nothing queued earlier may still read the buffers overwritten below
+    drain(s);
     nb::Dev &d = s->d;
     const size_t n = (size_t)d.n;
+    int r = NB_OK;
     {
         std::vector<float4> pm(n), vel(n), zero(n, make_float4(0.f, 0.f, 0.f, 0.f));
         for (size_t i = 0; i < n; ++i) {
             pm[i] = make_float4(x[i], y[i], z[i], m[i]);
             vel[i] = make_float4(vx[i], vy[i], vz[i], 0.f);
         }
-        HIP_TRY(s, hipMemcpy(d.pm4, pm.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        HIP_TRY(s, hipMemcpy(d.vel4, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        HIP_TRY(s, hipMemcpy(d.acc4, zero.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        r = upload(s, d.pm4, pm.data(), n * sizeof(float4));
+        if (!r) r = upload(s, d.vel4, vel.data(), n * sizeof(float4));
+        if (!r) r = upload(s, d.acc4, zero.data(), n * sizeof(float4));
+        if (r) return r;
     }
     if (d.fp64) {
         std::vector<double4> p64(n);
         for (size_t i = 0; i < n; ++i) p64[i] = make_double4(x[i], y[i], z[i], 0.0);
-        HIP_TRY(s, hipMemcpy(d.pos64, p64.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        r = upload(s, d.pos64, p64.data(), n * sizeof(double4));
+        if (r) return r;
     }
     // the build inserts bodies in the previous step's sorted order; before the
     // first step that order is the identity
     {
         std::vector<int> ident((size_t)d.n);
         for (int i = 0; i < d.n; ++i) ident[i] = i;
-        HIP_TRY(s, hipMemcpy(d.sorted, ident.data(), (size_t)d.n * sizeof(int), hipMemcpyHostToDevice));
+        r = upload(s, d.sorted, ident.data(), (size_t)d.n * sizeof(int));
+        if (r) return r;
     }
     // keep the epoch running so that no slot of an earlier tree can look current
     nb::Ctl cur;
-    int r = read_ctl(s, &cur);
+    r = read_ctl(s, &cur);
     if (r) return r;
     nb::Ctl c = initial_ctl(s, cur.epoch);
     HIP_TRY(s, hipMemcpyAsync(d.ctl, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
@@ -763,6 +803,8 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     s->inter_before = 0;
     s->cell_inter_before = 0;
     s->hint_valid = false; // the first step reads its level sizes back
+    s->ord_ready = false;  // the launch order of the coarse traversal belongs to the old bodies
+    reset_control(s);
     s->have_state = true;
     s->err.clear();
     return NB_OK;
@@ -778,6 +820,8 @@ int nb_run_stage(nb_sim *s, int32_t stage)
     return check_device_error(s);
 }
 
+static int step_loop(nb_sim *s, int32_t nsteps);
+
 int nb_step(nb_sim *s, int32_t nsteps)
 {
     if (!s) return NB_E_ARG;
@@ -786,6 +830,15 @@ int nb_step(nb_sim *s, int32_t nsteps)
     HIP_TRY(s, hipSetDevice(s->device));
     if (force_method(s) == NB_MODE_LEARNED && !s->have_model)
         return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
+    const int r = step_loop(s, nsteps);
+    // a failed step may have left launches queued on any of the streams: none
+    // may still run when the call returns
+    if (r) drain(s);
+    return r;
+}
+
+static int step_loop(nb_sim *s, int32_t nsteps)
+{
     // with timing on, every stage of every step is bracketed by events on the
     // handle's stream; stage_ms reports the per-step average
     const size_t nev = s->timing ? (size_t)7 * nsteps : 0;
@@ -840,23 +893,17 @@ int nb_copy_vertices(nb_sim *s, float *pos4, float *vel4)
 {
     if (!s) return NB_E_ARG;
     if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    if (!pos4 && !vel4) return NB_OK;
     HIP_TRY(s, hipSetDevice(s->device));
-    const size_t bytes = (size_t)s->d.n * sizeof(float4);
-    float4 *dp = nullptr, *dv = nullptr;
-    if (pos4) HIP_TRY(s, hipMalloc((void **)&dp, bytes));
-    if (vel4) {
-        hipError_t e = hipMalloc((void **)&dv, bytes);
-        if (e != hipSuccess) {
-            if (dp) (void)hipFree(dp);
-            return fail(s, NB_E_HIP, "hipMalloc: %s", hipGetErrorString(e));
-        }
-    }
+    const size_t n = (size_t)s->d.n, bytes = n * sizeof(float4);
+    // the staging buffer is taken once and reused by every later call (the
+    // renderer calls this after every step)
+    if (!s->vtx) HIP_TRY(s, dalloc(&s->vtx, 2 * n));
+    float4 *dp = pos4 ? s->vtx : nullptr, *dv = vel4 ? s->vtx + n : nullptr;
     hipError_t e = nb::launch_copy_vertices(s->d, dp, dv, s->stream);
     if (e == hipSuccess && dp) e = hipMemcpyAsync(pos4, dp, bytes, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess && dv) e = hipMemcpyAsync(vel4, dv, bytes, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (dp) (void)hipFree(dp);
-    if (dv) (void)hipFree(dv);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "copy_vertices: %s", hipGetErrorString(e));
     return NB_OK;
 }
@@ -1193,7 +1240,8 @@ int nb_train(nb_sim *s, const nb_train_params *tp_in, nb_train_report *rep)
         std::sort(perm.begin(), perm.begin() + G);
         std::copy(perm.begin(), perm.begin() + G, groups.begin() + (size_t)k * G);
     }
-    HIP_TRY(s, hipMemcpy(gdev, groups.data(), groups.size() * sizeof(int), hipMemcpyHostToDevice));
+    r = upload(s, gdev, groups.data(), groups.size() * sizeof(int));
+    if (r) return r;
 
     const auto t0 = std::chrono::steady_clock::now();
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
@@ -1504,12 +1552,14 @@ int nb_restore_state(nb_sim *s, const double *x, const double *y, const double *
     {
         std::vector<float4> acc(n);
         for (size_t i = 0; i < n; ++i) acc[i] = make_float4(ax[i], ay[i], az[i], 0.f);
-        HIP_TRY(s, hipMemcpy(d.acc4, acc.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        r = upload(s, d.acc4, acc.data(), n * sizeof(float4));
+        if (r) return r;
     }
     if (d.fp64) {
         std::vector<double4> p64(n);
         for (size_t i = 0; i < n; ++i) p64[i] = make_double4(x[i], y[i], z[i], 0.0);
-        HIP_TRY(s, hipMemcpy(d.pos64, p64.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        r = upload(s, d.pos64, p64.data(), n * sizeof(double4));
+        if (r) return r;
     }
     HIP_TRY(s, hipMemcpyAsync(&d.ctl->step, &step, sizeof(int), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, hipMemcpyAsync(&d.ctl->max_depth, &max_depth, sizeof(int), hipMemcpyHostToDevice, s->stream));
@@ -1539,6 +1589,39 @@ int nb_get_counters(nb_sim *s, nb_counters *out)
     out->check_ms = s->check_ms;
     out->checks = s->checks;
     out->coarse_alpha = s->alpha_now;
+    return NB_OK;
+}
+
+int nb_get_control(nb_sim *s, nb_control *out)
+{
+    if (!s || !out) return NB_E_ARG;
+    std::memset(out, 0, sizeof *out);
+    out->steps_since_model = s->steps_since_model;
+    out->check_seed = s->check_seed;
+    out->coarse_alpha = s->alpha_now;
+    out->fell_back = s->fell_back ? 1 : 0;
+    out->fail_streak = s->fail_streak;
+    out->fallbacks = s->fallbacks;
+    out->checks = s->checks;
+    out->last_check_median = s->last_check_median;
+    return NB_OK;
+}
+
+int nb_set_control(nb_sim *s, const nb_control *in)
+{
+    if (!s || !in) return NB_E_ARG;
+    if (in->steps_since_model < 0 || !(in->coarse_alpha >= 0.0f) || !std::isfinite(in->coarse_alpha) ||
+        (in->fell_back != 0 && in->fell_back != 1) || in->fail_streak < 0 || in->fallbacks < 0 || in->checks < 0)
+        return fail(s, NB_E_ARG, "invalid control state");
+    s->steps_since_model = in->steps_since_model;
+    s->check_seed = in->check_seed;
+    s->alpha_now = in->coarse_alpha;
+    s->d.alpha2_coarse = in->coarse_alpha > 0.0f ? (float)((double)in->coarse_alpha * (double)in->coarse_alpha) : 0.0f;
+    s->fell_back = in->fell_back != 0;
+    s->fail_streak = in->fail_streak;
+    s->fallbacks = in->fallbacks;
+    s->checks = in->checks;
+    s->last_check_median = in->last_check_median;
     return NB_OK;
 }
 

@@ -105,6 +105,21 @@ class NbTrainReport(C.Structure):
     ]
 
 
+class NbControl(C.Structure):
+    _fields_ = [
+        ("steps_since_model", C.c_int64),
+        ("check_seed", C.c_uint64),
+        ("coarse_alpha", C.c_float),
+        ("fell_back", C.c_int32),
+        ("fail_streak", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("fallbacks", C.c_int64),
+        ("checks", C.c_int64),
+        ("last_check_median", C.c_double),
+        ("reserved", C.c_int64 * 4),
+    ]
+
+
 class NbError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[{code}] {message}")
@@ -163,6 +178,8 @@ def load() -> C.CDLL:
     L.nb_diagnostics.argtypes = [vp, C.c_int32, C.c_uint64, vp]
     L.nb_get_masses.argtypes = [vp, vp]
     L.nb_restore_state.argtypes = [vp] + [vp] * 10 + [C.c_int32, C.c_int32]
+    L.nb_get_control.argtypes = [vp, C.POINTER(NbControl)]
+    L.nb_set_control.argtypes = [vp, C.POINTER(NbControl)]
     L.nb_last_error.argtypes = [vp]
     L.nb_last_error.restype = C.c_char_p
     L.nb_status_string.argtypes = [C.c_int]
@@ -173,7 +190,7 @@ def load() -> C.CDLL:
                  "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error",
                  "nb_train", "nb_set_model", "nb_get_model", "nb_coarse_features",
                  "nb_mlp_forward", "nb_mlp_loss_grad", "nb_mlp_adam", "nb_diagnostics",
-                 "nb_get_masses", "nb_restore_state"):
+                 "nb_get_masses", "nb_restore_state", "nb_get_control", "nb_set_control"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

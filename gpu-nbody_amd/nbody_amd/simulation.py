@@ -336,15 +336,35 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
         _check(self._lib, self._h, self._lib.nb_get_masses(self._h, ptr(m)))
         return m
 
+    def control(self) -> "_lib.NbControl":
+        """Learned-mode control state (check schedule, sample seed, alpha in
+        effect, fallback)."""
+        c = _lib.NbControl()
+        _check(self._lib, self._h, self._lib.nb_get_control(self._h, C.byref(c)))
+        return c
+
+    def set_control(self, c: "_lib.NbControl") -> None:
+        _check(self._lib, self._h, self._lib.nb_set_control(self._h, C.byref(c)))
+
     def checkpoint(self, path: str) -> None:
         """Write the full dynamic state (npz): positions (FP64 when kept), velocities,
-        last accelerations, masses, step counter and running maximum depth."""
+        last accelerations, masses, step counter and running maximum depth; in the
+        learned mode also the model blob and the control state (nb_control)."""
         c = self.counters()
         x, y, z, vx, vy, vz = self.state()
         pos = (self.positions64() if self._params.flags & _lib.NB_FP64_POSITIONS
                else np.stack([x, y, z], 1).astype(np.float64))
+        extra = {}
+        if self._params.mode == _lib.NB_MODE_LEARNED:
+            try:
+                extra["model"] = self.get_model()
+            except NbError as e:
+                if e.code != _lib.NB_E_MODEL:
+                    raise
+            ctl = self.control()
+            extra["control"] = np.frombuffer(bytes(ctl), np.uint8).copy()
         np.savez(path, pos=pos, vel=np.stack([vx, vy, vz], 1), acc=self.accel(), m=self.masses(),
-                 step=np.int32(c.step), max_depth=np.int32(c.max_depth))
+                 step=np.int32(c.step), max_depth=np.int32(c.max_depth), **extra)
 
     def restore(self, path: str) -> None:
         """Resume from a checkpoint written by checkpoint()."""
@@ -356,3 +376,10 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
         cols.append(np.ascontiguousarray(f["m"], dtype=np.float32))
         _check(self._lib, self._h, self._lib.nb_restore_state(self._h, *[ptr(a) for a in cols],
                                                               int(f["step"]), int(f["max_depth"])))
+        if "model" in f.files:
+            self.set_model(f["model"])
+        if "control" in f.files:
+            raw = f["control"].tobytes()
+            if len(raw) != C.sizeof(_lib.NbControl):
+                raise ValueError("checkpoint control record has the wrong size")
+            self.set_control(_lib.NbControl.from_buffer_copy(raw))

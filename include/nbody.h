@@ -99,7 +99,8 @@ void nb_destroy(nb_sim *sim);
 /* UniverseGenerator.generate(0, n, x, y, z, vx, vy, vz, m) followed by the
  * buffer upload of init (UniverseGenerator.java:22, GPUBarnesHutNBodySimulation.java:134-146).
  * Host SoA arrays of n_bodies floats each. Resets the step counter to -1,
- * accelerations to 0 and the running maximum depth to 1 (java:163-170). */
+ * accelerations to 0 and the running maximum depth to 1 (java:163-170), and
+ * the learned mode's control state (nb_control) to a new handle's. */
 int nb_set_state(nb_sim *sim, const float *x, const float *y, const float *z,
                  const float *vx, const float *vy, const float *vz, const float *m);
 
@@ -123,7 +124,9 @@ int nb_run_stage(nb_sim *sim, int32_t stage);
 /* copyVertices (copyvertices.cl:7-17, launched at java:265-266): writes
  * n_bodies float4 (x, y, z, 1) into pos4 and (vx, vy, vz, 1) into vel4
  * (either may be NULL). Replaces the CL-GL shared VBOs of initGLBuffers
- * (java:229-246): the caller uploads them into its own VBOs. */
+ * (java:229-246): the caller uploads them into its own VBOs. The first call
+ * allocates a device staging buffer of 32 n_bodies bytes that later calls
+ * reuse (freed by nb_destroy). */
 int nb_copy_vertices(nb_sim *sim, float *pos4, float *vel4);
 
 /* getNumberOfBodies() (AbstractNBodySimulation.java:32, java:377-379). */
@@ -292,11 +295,37 @@ int nb_get_masses(nb_sim *sim, float *m);
 
 /* Resume from a checkpoint: positions (FP64; rounded to FP32 for the tree),
  * velocities, last accelerations and masses by original body index, and the
- * step counter and running maximum depth (nb_get_counters). The next nb_step
- * continues exactly as the interrupted run would have. */
+ * step counter and running maximum depth (nb_get_counters). In plain and
+ * coarse modes the next nb_step continues exactly as the interrupted run
+ * would have; the learned mode also needs its model and nb_control back
+ * (nb_set_model, then nb_set_control). */
 int nb_restore_state(nb_sim *sim, const double *x, const double *y, const double *z, const float *vx,
                      const float *vy, const float *vz, const float *ax, const float *ay, const float *az,
                      const float *m, int32_t step, int32_t max_depth);
+
+/* Learned-mode control state: what, beside the bodies (nb_restore_state) and
+ * the model (nb_get_model / nb_set_model), decides how the next steps of a
+ * learned run go -- the sampled check's schedule and sample generator, the
+ * relative test's alpha after check_adapt tightened it, and a fallback to
+ * plain theta. A checkpoint saves it after the model; a resume calls
+ * nb_restore_state, nb_set_model, then nb_set_control (nb_set_state and
+ * nb_set_model put it back to a new handle's values). No reference
+ * counterpart: the reference has no learned mode and no checkpoint. */
+typedef struct nb_control {
+    int64_t steps_since_model; /* learned steps since the model was installed: a step
+                                  runs the check when this is a multiple of check_every */
+    uint64_t check_seed;       /* state of the generator that draws the checked bodies */
+    float coarse_alpha;        /* the relative test's alpha in effect */
+    int32_t fell_back;         /* 1: a failed check switched the force to plain theta */
+    int32_t fail_streak;       /* failed checks in a row */
+    int32_t reserved0;
+    int64_t fallbacks;         /* counters carried into nb_get_counters: fallback steps, */
+    int64_t checks;            /* checks run, */
+    double last_check_median;  /* and the last check's median relative error */
+    int64_t reserved[4];
+} nb_control;
+int nb_get_control(nb_sim *sim, nb_control *out);
+int nb_set_control(nb_sim *sim, const nb_control *in);
 
 /* Per-stage HIP-event timing of the following nb_step calls (0 = off). */
 int nb_set_timing(nb_sim *sim, int32_t enabled);

@@ -525,6 +525,22 @@ int orc_force_groups(orc_sim *s, const int *groups, int ngroups)
     return ORC_OK;
 }
 
+/* The CPU baseline's FP32 force pass over the listed vote groups (cpu_fp32.c:
+ * the reference's FP32 arithmetic, calculateforce.cl:143-151, vectorized over
+ * a group's lanes). Accelerations by sorted position into ax/ay/az (n each);
+ * the oracle's own accelerations are not touched. Returns the interactions. */
+long long cpu_force_fp32(int n, int nn, const float *X, const float *Y, const float *Z, const float *M,
+                         const int *child, const int *sorted, const float *dq, float eps2, int vote,
+                         const int *groups, int ngroups, float *ax_out, float *ay_out, float *az_out);
+
+long long orc_force_fp32_groups(orc_sim *s, const int *groups, int ngroups, float *ax, float *ay, float *az)
+{
+    orc_dq_table(s->radius, s->theta2, s->eps2, s->max_depth, s->dq);
+    if (s->max_depth > ORC_MAXDEPTH) return -1;
+    return cpu_force_fp32(s->n, s->nn, s->x, s->y, s->z, s->m, s->child, s->sorted, s->dq, s->eps2, s->vote,
+                          groups, ngroups, ax, ay, az);
+}
+
 /* ---- stage 6: integrate, integrate.cl:27-44 (kick, drift, kick with the
  * acceleration just computed). ------------------------------------------ */
 void orc_integrate(orc_sim *s)

@@ -60,6 +60,8 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.orc_force_groups.argtypes = [vp, _i32p, C.c_int]
         L.orc_force_groups.restype = C.c_int
+        L.orc_force_fp32_groups.argtypes = [vp, _i32p, C.c_int, _f32p, _f32p, _f32p]
+        L.orc_force_fp32_groups.restype = C.c_longlong
         L.orc_step.argtypes = [vp, C.c_int]
         L.orc_step.restype = C.c_int
         L.orc_run_stages.argtypes = [vp, C.c_int, C.c_int]
@@ -184,6 +186,19 @@ class Oracle:
     def force_groups(self, groups):
         g = np.ascontiguousarray(groups, dtype=np.int32)
         return lib().orc_force_groups(self.h, g, len(g))
+
+    def force_fp32(self, groups=None):
+        """The CPU baseline's FP32 force pass (cpu_fp32.c) over the listed vote
+        groups (all when None), on this oracle's tree: (accelerations by sorted
+        position [n, 3], rows of unlisted groups 0; interactions)."""
+        if groups is None:
+            groups = np.arange((self.n + self.vote - 1) // self.vote, dtype=np.int32)
+        g = np.ascontiguousarray(groups, dtype=np.int32)
+        a = [np.zeros(self.n, np.float32) for _ in range(3)]
+        ni = lib().orc_force_fp32_groups(self.h, g, len(g), *a)
+        if ni < 0:
+            raise RuntimeError("orc_force_fp32_groups rejected the configuration")
+        return np.stack(a, axis=1), int(ni)
 
     def state(self):
         arrs = [np.zeros(self.n, np.float32) for _ in range(6)]

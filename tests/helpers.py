@@ -16,6 +16,20 @@ def make_sim(state, **kw):
     return sim
 
 
+def random_model(seed, scale_out=1e-3):
+    """A seeded model blob (network parameters, feature and target scalings)
+    whose corrections are about scale_out of the coarse acceleration."""
+    rng = np.random.default_rng(seed)
+    P = np.concatenate([rng.uniform(-1, 1, 24 * 64) / np.sqrt(24), rng.uniform(-1, 1, 64) / np.sqrt(24),
+                        rng.uniform(-1, 1, 64 * 64) / 8, rng.uniform(-1, 1, 64) / 8,
+                        rng.uniform(-1, 1, 64 * 3) / 8, rng.uniform(-1, 1, 3) / 8])
+    xmu = rng.normal(size=24)
+    xsd = rng.uniform(0.5, 2.0, 24)
+    ymu = rng.normal(size=3) * scale_out
+    ysd = rng.uniform(0.5, 2.0, 3) * scale_out
+    return np.concatenate([P, xmu, xsd, ymu, ysd]).astype(np.float32)
+
+
 def make_oracle(state, theta, softening, dt, vote, flags=0):
     import pyoracle
 

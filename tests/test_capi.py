@@ -97,8 +97,19 @@ def test_null_handles(nb):
     assert lib.nb_copy_vertices(None, None, None) == E
     assert lib.nb_num_bodies(None) == 0
     assert lib.nb_get_counters(None, None) == E
+    assert lib.nb_get_control(None, None) == E
+    assert lib.nb_set_control(None, None) == E
     assert lib.nb_create(None, None) == E
     lib.nb_destroy(None)
+
+
+def test_control_record_layout(nb):
+    """nb_control as the header lays it out: 8-byte fields first, 88 bytes."""
+    import ctypes as C
+
+    c = nb._lib.NbControl
+    assert C.sizeof(c) == 88
+    assert c.coarse_alpha.offset == 16 and c.fallbacks.offset == 32 and c.last_check_median.offset == 48
 
 
 def test_status_strings(nb):

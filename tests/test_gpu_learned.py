@@ -22,7 +22,7 @@ Bars:
 import numpy as np
 import pytest
 
-from helpers import make_oracle, make_sim, rel_err
+from helpers import make_oracle, make_sim, random_model as _random_model, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -36,18 +36,6 @@ def _coarse_sim(st, method=None, **kw):
 
     return make_sim(st, theta=0.4, softening=EPS, dt=1e-3, vote_width=64, coarse_theta=THETA_C,
                     method=L.NB_MODE_COARSE if method is None else method, **kw)
-
-
-def _random_model(seed, scale_out=1e-3):
-    rng = np.random.default_rng(seed)
-    P = np.concatenate([rng.uniform(-1, 1, 24 * 64) / np.sqrt(24), rng.uniform(-1, 1, 64) / np.sqrt(24),
-                        rng.uniform(-1, 1, 64 * 64) / 8, rng.uniform(-1, 1, 64) / 8,
-                        rng.uniform(-1, 1, 64 * 3) / 8, rng.uniform(-1, 1, 3) / 8])
-    xmu = rng.normal(size=24)
-    xsd = rng.uniform(0.5, 2.0, 24)
-    ymu = rng.normal(size=3) * scale_out
-    ysd = rng.uniform(0.5, 2.0, 3) * scale_out
-    return np.concatenate([P, xmu, xsd, ymu, ysd]).astype(np.float32)
 
 
 def _hand_over(sim, st, step_offset=0):
@@ -92,6 +80,11 @@ def test_coarse_features_match_oracle(nb, oracle, case):
     ga, gf, ge2 = a[srt], f[srt], e2[srt]
     err = rel_err(ga, oa)
     assert err.max() <= 1e-5, (err.max(), int(err.argmax()))
+    # features 17 and 18 are logs of integer counts: the counts are the oracle's exactly
+    for k in (17, 18):
+        cg = np.rint(np.expm1(gf[:, k].astype(np.float64)))
+        co = np.rint(np.expm1(of[:, k].astype(np.float64)))
+        assert np.array_equal(cg, co), (k, np.argwhere(cg != co)[:5])
     bad = np.abs(gf - of) > 1e-3 + 1e-3 * np.abs(of)
     rows_bad = bad.any(axis=1)
     assert rows_bad.sum() <= max(2, n // 2000), (rows_bad.sum(), np.argwhere(bad)[:10])

@@ -22,7 +22,13 @@ counterpart, so the oracle is checked against what the mathematics fixes:
               sums that vanish: sum |a| (f5 e1 + f6 e2 + f7 e3) = sum (a - mean a)
               and sum (f8 e1 + f9 e2 + f10 e3) = sum (x - centroid) / r_g; and
               the group means taken back out of each body's frame, the same
-              vector on every lane and equal to the mean of the per-body parts
+              vector on every lane and equal to the mean of the per-body parts;
+              the counts (17, 18) sum to the traversal's interaction count, are
+              the same on every lane of a group, and give the bodies met that
+              feature 21's mass split implies at equal masses; exp(2 f20) =
+              f0^2 + f1^2 + f2^2; feature 22 against a brute-force group
+              radius; feature 23 against the cosine of the monopole part left
+              in the returned acceleration
   network     wiring by one-path networks (a transposed or misindexed weight
               moves the output), GELU at known points, and the backward pass
               against central finite differences of the forward pass
@@ -236,6 +242,60 @@ def test_features_mass_scaling_invariance_and_identities(oracle):
         gm = (back(f1[:, cols].astype(np.float64)) * A).reshape(-1, 64, 3)
         scale = np.linalg.norm(part.mean(1), axis=1).max() + 1e-300
         assert np.abs(gm - part.mean(1, keepdims=True)).max() <= 1e-5 * scale
+
+
+def test_feature_counts_radius_and_cosine(oracle):
+    """Features 17, 18, 20, 22 and 23 against what fixes them from outside
+    the feature code: the interaction count the traversal returns, the mass
+    split of feature 21 (equal masses: bodies met = N (1 - f21)), a
+    brute-force group radius, the body part (features 0-2), and the monopole
+    part left when the body and quadrupole parts are taken out of the
+    returned coarse acceleration."""
+    from nbody_amd import universe as U
+
+    st = U.plummer(4096, seed=6)  # equal masses 1/N
+    n = len(st[0])
+    o = oracle.Oracle(n, theta=0.5, softening=0.01, dt=1e-3, vote=64, flags=1)
+    o.set_state(*st)
+    assert o.run_stages(1, 4) == 0
+    ap, _, _, _ = o.coarse(0.5, features=False, geometric=True)
+    srt = o.sorted()
+    prev = np.zeros((n, 3), np.float32)
+    prev[srt] = ap[:n]
+    o.set_accel(*(np.ascontiguousarray(prev[:, j]) for j in range(3)), 1)
+    assert o.run_stages(1, 4) == 0
+    srt = o.sorted()
+    a, f, e2, ninter = o.coarse(0.9, alpha=0.02)
+    f = f.astype(np.float64)
+    # 17, 18: integer counts, the same on every lane of a vote group (the
+    # group takes a cell or opens it as a whole), summing to the interactions
+    nc = np.expm1(f[:, 17])
+    nb = np.expm1(f[:, 18])
+    assert np.abs(nc - np.rint(nc)).max() < 1e-2 and np.abs(nb - np.rint(nb)).max() < 1e-2
+    nc, nb = np.rint(nc).astype(np.int64), np.rint(nb).astype(np.int64)
+    assert (nc.reshape(-1, 64) == nc.reshape(-1, 64)[:, :1]).all()
+    assert (nb.reshape(-1, 64) == nb.reshape(-1, 64)[:, :1]).all()
+    assert int((nc + nb).sum()) == ninter
+    assert np.abs(nb - n * (1.0 - f[:, 21])).max() < 0.05
+    assert (nc > 0).all() and (nb > 0).all()
+    # 20: |a_b| / |a| from the frame components of a_b
+    assert np.allclose(np.exp(2 * f[:, 20]), (f[:, 0:3] ** 2).sum(1), rtol=1e-4, atol=0)
+    # 22: the group's radius about its centroid over the root's radius
+    P = np.stack(st[:3], 1).astype(np.float64)[srt].reshape(-1, 64, 3)
+    cen = P.mean(1, keepdims=True)
+    rg = np.sqrt(((P - cen) ** 2).sum(2)).max(1)
+    radius = float(o.root()[3])
+    assert np.allclose(np.exp(f[:, 22]).reshape(-1, 64), (rg / radius)[:, None], rtol=1e-5, atol=0)
+    # 23: cosine between the cell monopole part and the body part
+    A = np.linalg.norm(a, axis=1, keepdims=True)
+    e1 = a / A
+    e3 = np.cross(e1, e2)
+    ab = (f[:, 0:1] * e1 + f[:, 1:2] * e2 + f[:, 2:3] * e3) * A
+    aq = (f[:, 3:4] * e1 + f[:, 4:5] * e2) * A
+    mono = a - ab - aq
+    cos = (mono * ab).sum(1) / (np.linalg.norm(mono, axis=1) * np.linalg.norm(ab, axis=1))
+    assert np.abs(cos - f[:, 23]).max() < 1e-4, np.abs(cos - f[:, 23]).max()
+    assert f[:, 23].min() >= -1.0 and f[:, 23].max() <= 1.0
 
 
 # ---- network ---------------------------------------------------------------

@@ -406,3 +406,31 @@ def test_nodes_for_reference_sizes(oracle):
     assert oracle.nodes_for(1000) == 16384
     assert oracle.nodes_for(4194304) == 8388608
     assert oracle.nodes_for(8193) == 16400
+
+
+@pytest.mark.parametrize("vote", [16, 64])
+def test_cpu_baseline_fp32_force_is_the_same_traversal(oracle, vote):
+    """The CPU baseline's FP32 force pass (oracle/cpu_fp32.c, timed beside the
+    GPU) takes the same opening decisions as the oracle (identical interaction
+    count) and differs from the FP64-accumulated oracle only by FP32 rounding;
+    on a subset of groups it leaves the other rows alone."""
+    from nbody_amd import universe as U
+
+    st = U.plummer(20000, seed=41)
+    o = oracle.Oracle(len(st[0]), theta=0.5, softening=0.05, dt=1e-3, vote=vote)
+    o.set_state(*st)
+    assert o.run_stages(1, 5) == 0
+    srt = o.sorted()
+    ref = o.accel()[srt].astype(np.float64)
+    a, ni = o.force_fp32()
+    assert ni == o.scalars()["interactions"]
+    err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() < 2e-5, err.max()
+    groups = np.array([0, 3, 7], np.int32)
+    a2, ni2 = o.force_fp32(groups)
+    rows = (groups[:, None] * vote + np.arange(vote)[None, :]).ravel()
+    rows = rows[rows < len(st[0])]
+    assert np.array_equal(a2[rows], a[rows])
+    mask = np.ones(len(st[0]), bool)
+    mask[rows] = False
+    assert not a2[mask].any() and 0 < ni2 < ni
