@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -78,7 +79,21 @@ struct nb_sim {
     std::string err;
 };
 
+// Staging of nb_copy_vertices_noalloc: a file-scope device array (one copy on
+// each GPU, held until the process ends), so that entry point allocates
+// nothing. Bodies go through it kNoAllocChunk at a time: positions in the
+// first half, velocities in the second (2 MiB in all).
+constexpr int kNoAllocChunk = 65536;
+static __device__ __align__(256) float4 scratch[2 * kNoAllocChunk];
+
 namespace {
+
+// Device address of `scratch` on each GPU, looked up once per device by the
+// first nb_create on it (host code never passes the array's own name, which
+// is a host-side stand-in, not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float4 *g_scratch_ptr[kMaxDevices] = {};
 
 // message of the last nb_create that failed on this thread (nb_last_error(NULL))
 thread_local std::string g_create_error;
@@ -338,6 +353,28 @@ int ensure_learn(nb_sim *s)
 {
     if (!s->mlpbuf) HIP_TRY(s, hipMalloc(&s->mlpbuf, nb::kMlpBufBytes));
     return ensure_quad(s);
+}
+
+// With the handle's device current: the device address of `scratch` there,
+// looked up on the first call for that device.
+int init_scratch(nb_sim *s)
+{
+    if (s->device < 0 || s->device >= kMaxDevices)
+        return fail(s, NB_E_ARG, "device %d is beyond the %d devices of the staging table", s->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[s->device]) return NB_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(s, NB_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[s->device] = static_cast<float4 *>(p);
+    return NB_OK;
+}
+
+float4 *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
 }
 
 // Waits for everything queued on the handle's streams (the main stream, the
@@ -732,6 +769,7 @@ int nb_create(const nb_params *p, nb_sim **out)
         return bail(NB_E_HIP);
     }
     if (d.quad && ensure_learn(s) != NB_OK) return bail(NB_E_HIP);
+    if (const int rs = init_scratch(s)) return bail(rs);
     // epoch 0 tags every slot as stale; the first bounding box moves to epoch 1
     ok = hipMemsetAsync(d.slot, 0, 8 * cap * sizeof(unsigned long long), s->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(d.cinfo, 0xff, cap * sizeof(int4), s->stream) == hipSuccess;
@@ -900,11 +938,38 @@ int nb_copy_vertices(nb_sim *s, float *pos4, float *vel4)
     // renderer calls this after every step)
     if (!s->vtx) HIP_TRY(s, dalloc(&s->vtx, 2 * n));
     float4 *dp = pos4 ? s->vtx : nullptr, *dv = vel4 ? s->vtx + n : nullptr;
-    hipError_t e = nb::launch_copy_vertices(s->d, dp, dv, s->stream);
+    hipError_t e = nb::launch_copy_vertices(s->d, 0, (int)n, dp, dv, s->stream);
     if (e == hipSuccess && dp) e = hipMemcpyAsync(pos4, dp, bytes, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess && dv) e = hipMemcpyAsync(vel4, dv, bytes, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return fail(s, NB_E_HIP, "copy_vertices: %s", hipGetErrorString(e));
+    return NB_OK;
+}
+
+int nb_copy_vertices_noalloc(nb_sim *s, float *pos4, float *vel4)
+{
+    if (!s) return NB_E_ARG;
+    if (!s->have_state) return fail(s, NB_E_STATE, "nb_set_state has not been called");
+    if (!pos4 && !vel4) return NB_OK;
+    HIP_TRY(s, hipSetDevice(s->device));
+    float4 *stage = scratch_ptr(s->device);
+    if (!stage) return fail(s, NB_E_STATE, "no staging address for device %d", s->device);
+    const int n = s->d.n;
+    // one chunk at a time through the staging array; the next chunk's kernel
+    // is ordered after the previous chunk's copies on the same stream
+    for (int first = 0; first < n; first += kNoAllocChunk) {
+        const int cnt = std::min(kNoAllocChunk, n - first);
+        const size_t bytes = (size_t)cnt * sizeof(float4);
+        float4 *dp = pos4 ? stage : nullptr, *dv = vel4 ? stage + kNoAllocChunk : nullptr;
+        hipError_t e = nb::launch_copy_vertices(s->d, first, cnt, dp, dv, s->stream);
+        if (e == hipSuccess && dp) e = hipMemcpyAsync(pos4 + 4 * (size_t)first, dp, bytes, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess && dv) e = hipMemcpyAsync(vel4 + 4 * (size_t)first, dv, bytes, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(s->stream);
+            return fail(s, NB_E_HIP, "copy_vertices_noalloc: %s", hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
     return NB_OK;
 }
 

@@ -138,7 +138,8 @@ hipError_t launch_integrate(const Dev &d, hipStream_t st);
 // FP64 per-block sums of m, m v, m v^2 / 2, m x cross v, m x (11 doubles per block)
 constexpr int kMomentBlocks = 512;
 hipError_t launch_moments(const Dev &d, double *part, hipStream_t st);
-hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st);
+// bodies first .. first + count - 1 into pos[0 .. count) / vel[0 .. count) (either may be null)
+hipError_t launch_copy_vertices(const Dev &d, int first, int count, float4 *pos, float4 *vel, hipStream_t st);
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st);
 
 // ---- learned coarse-theta correction (nb_learned.hip) -----------------------

@@ -771,17 +771,20 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d)
 }
 
 // copyVertices (copyvertices.cl:7-17)
-__global__ __launch_bounds__(256) void k_copy_vertices(Dev d, float4 *pos, float4 *vel)
+// Bodies first .. first + count - 1; the outputs are indexed from `first`
+// (pos[i - first]), so a range can go into a staging buffer of its own size.
+__global__ __launch_bounds__(256) void k_copy_vertices(Dev d, int first, int count, float4 *pos, float4 *vel)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const int i = first + j;
     if (pos) {
         const float4 p = d.pm4[i];
-        pos[i] = make_float4(p.x, p.y, p.z, 1.0f);
+        pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
     }
     if (vel) {
         const float4 v = d.vel4[i];
-        vel[i] = make_float4(v.x, v.y, v.z, 1.0f);
+        vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
     }
 }
 
@@ -877,9 +880,10 @@ hipError_t launch_integrate(const Dev &d, hipStream_t st)
     hipLaunchKernelGGL(k_integrate, dim3((d.n + 255) / 256), dim3(256), 0, st, d);
     return hipGetLastError();
 }
-hipError_t launch_copy_vertices(const Dev &d, float4 *pos, float4 *vel, hipStream_t st)
+hipError_t launch_copy_vertices(const Dev &d, int first, int count, float4 *pos, float4 *vel, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_copy_vertices, dim3((d.n + 255) / 256), dim3(256), 0, st, d, pos, vel);
+    if (first < 0 || count < 1 || first > d.n - count) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_vertices, dim3((count + 255) / 256), dim3(256), 0, st, d, first, count, pos, vel);
     return hipGetLastError();
 }
 hipError_t launch_paths(const Dev &d, unsigned long long *paths3, int *depth, hipStream_t st)

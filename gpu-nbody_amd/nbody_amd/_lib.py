@@ -152,6 +152,7 @@ def load() -> C.CDLL:
     L.nb_step.argtypes = [vp, C.c_int32]
     L.nb_run_stage.argtypes = [vp, C.c_int32]
     L.nb_copy_vertices.argtypes = [vp, vp, vp]
+    L.nb_copy_vertices_noalloc.argtypes = [vp, vp, vp]
     L.nb_num_bodies.argtypes = [vp]
     L.nb_num_bodies.restype = C.c_int32
     L.nb_get_state.argtypes = [vp] + [vp] * 6
@@ -185,6 +186,7 @@ def load() -> C.CDLL:
     L.nb_status_string.argtypes = [C.c_int]
     L.nb_status_string.restype = C.c_char_p
     for name in ("nb_create", "nb_set_state", "nb_step", "nb_run_stage", "nb_copy_vertices",
+                 "nb_copy_vertices_noalloc",
                  "nb_get_state", "nb_get_accel", "nb_get_positions64", "nb_get_root",
                  "nb_get_sorted", "nb_get_paths", "nb_get_cells", "nb_get_counters",
                  "nb_set_timing", "nb_probe_accel", "nb_direct_accel", "nb_force_error",

@@ -190,6 +190,11 @@ class HipBarnesHutNBodySimulation(AbstractNBodySimulation):
     def copy_vertices(self, pos4: Optional[np.ndarray], vel4: Optional[np.ndarray]) -> None:
         _check(self._lib, self._h, self._lib.nb_copy_vertices(self._h, ptr(pos4), ptr(vel4)))
 
+    def copy_vertices_noalloc(self, pos4: Optional[np.ndarray], vel4: Optional[np.ndarray]) -> None:
+        """copy_vertices without any device allocation (a staging array the
+        library holds per GPU)."""
+        _check(self._lib, self._h, self._lib.nb_copy_vertices_noalloc(self._h, ptr(pos4), ptr(vel4)))
+
     def state(self):
         n = self.nbodies
         a = [np.zeros(n, np.float32) for _ in range(6)]

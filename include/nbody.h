@@ -129,6 +129,15 @@ int nb_run_stage(nb_sim *sim, int32_t stage);
  * reuse (freed by nb_destroy). */
 int nb_copy_vertices(nb_sim *sim, float *pos4, float *vel4);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory -- code under HIP graph capture, per-frame render loops, harnesses
+ * that forbid hipMalloc: the output of nb_copy_vertices, staged 65,536 bodies
+ * at a time through a 2 MiB device array that the library holds on each GPU
+ * until the process ends (looked up by nb_create), so the call allocates
+ * nothing and has no size limit. Every handle on a GPU shares that array:
+ * calls that overlap on one GPU (from different threads) race. */
+int nb_copy_vertices_noalloc(nb_sim *sim, float *pos4, float *vel4);
+
 /* getNumberOfBodies() (AbstractNBodySimulation.java:32, java:377-379). */
 int32_t nb_num_bodies(const nb_sim *sim);
 
