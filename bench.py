@@ -554,7 +554,11 @@ def main():
                 "steps_per_s": round(c["steps_per_s"], 4),
                 "steps_per_s_median_of_3_runs": round(float(np.median(c["runs"])), 4),
                 "median_rel_force_err": round(c["err"]["median"], 8),
-                "learned_over_coarse_only": round(float(np.median(main_r["runs"]) / np.median(c["runs"])), 3)}
+                "learned_over_coarse_only": round(float(np.median(main_r["runs"]) / np.median(c["runs"])), 3),
+                # the coarse mode has no sampled check: the learned steps between checks beside it
+                "learned_between_checks_over_coarse_only": (
+                    round(main_r["steps_per_s_between_checks"] / float(np.median(c["runs"])), 3)
+                    if main_r["steps_per_s_between_checks"] else None)}
     if not args.no_cpu_baseline:
         host = host_cpu()
         log(f"cpu baseline on {host['threads']} threads ({host['model']})")

@@ -77,10 +77,20 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
         }
         float px = 0.f, py = 0.f, pz = 0.f;
         int cnt = 0;
+        int npush = 0;
+        bool badv = false;
+        unsigned push = 0;
+        // Every branch in this loop is wave-uniform and none leaves it early
+        // (children are compacted to the front: past the first negative entry
+        // every one is negative). An opened cell's stack entry goes into lane
+        // `npush` of `push` by a select, and the opened children are stored
+        // after the loop in one go: no lane-divergent store inside the loop,
+        // which otherwise makes the compiler copy the running sums at every
+        // child.
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int v = ch[k];
-            if (v < 0) break;
+            if (v < 0) continue;
             const float dx = q[k].x - me.x, dy = q[k].y - me.y, dz = q[k].z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             bool acc = v < n;
@@ -93,13 +103,20 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
                 pz = fmaf(dz, f, pz);
                 ++cnt;
             } else {
-                // the tree is at most kMaxDepth deep: anything else is corrupt
-                if (sp >= kStack || v - n >= cap || dep + 1 >= kMaxDepth) {
-                    bad = true;
-                    break;
-                }
-                if (lane == 0) stk[wv][sp] = ((unsigned)(v - n) << 6) | (unsigned)(dep + 1);
-                ++sp;
+                push = lane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
+                ++npush;
+                badv |= v - n >= cap;
+            }
+        }
+        // the opened children in child order (the last one is popped first);
+        // the tree is at most kMaxDepth deep and the pool holds `cap` cells:
+        // anything else is corrupt
+        if (npush) {
+            if (sp + npush > kStack || badv || dep + 1 >= kMaxDepth) {
+                bad = true;
+            } else {
+                if (lane < npush) stk[wv][sp + lane] = push;
+                sp += npush;
             }
         }
         kahan_add(sx, kx, px);
@@ -146,7 +163,8 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
     const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const unsigned long long gmask = ((1ull << G) - 1ull) << ((lane / G) * G);
-    const bool leader = (lane % G) == 0;
+    const int glane = lane % G; // this lane's place in its group
+    const bool leader = glane == 0;
 
     if (leader) stk[gl][0] = 0; // root cell 0 at depth 0
     int sp = 1;
@@ -165,10 +183,16 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
         const float4 *cdp = d.cdata + (size_t)c * 8;
         float px = 0.f, py = 0.f, pz = 0.f;
         int cnt = 0;
+        int npush = 0;
+        bool badv = false;
+        unsigned push = 0;
+        // as in k_force64: no early exit, and the opened children's stack
+        // entries gathered in lanes 0 .. npush-1 of the group by a select and
+        // stored after the loop
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int v = ch[k];
-            if (v < 0) break;
+            if (v < 0) continue;
             const float4 q = cdp[k];
             const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
@@ -182,12 +206,17 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
                 pz = fmaf(dz, f, pz);
                 ++cnt;
             } else {
-                if (sp >= kStack || v - n >= d.cap || dep + 1 >= kMaxDepth) {
-                    bad = true;
-                    break;
-                }
-                if (leader) stk[gl][sp] = ((unsigned)(v - n) << 6) | (unsigned)(dep + 1);
-                ++sp;
+                push = glane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
+                ++npush;
+                badv |= v - n >= d.cap;
+            }
+        }
+        if (npush) {
+            if (sp + npush > kStack || badv || dep + 1 >= kMaxDepth) {
+                bad = true;
+            } else {
+                if (glane < npush) stk[gl][sp + glane] = push;
+                sp += npush;
             }
         }
         kahan_add(sx, kx, px);
