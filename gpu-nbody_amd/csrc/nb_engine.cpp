@@ -27,7 +27,7 @@ struct nb_sim {
     int cus = 0;
     hipStream_t stream = nullptr;
     nb::Dev d{};
-    int bbox_blocks = 1, build_blocks = 1;
+    int bbox_blocks = 1;
     bool have_state = false;
     bool timing = false;
     std::vector<hipEvent_t> tev; // per-step stage events (timing mode)
@@ -118,21 +118,12 @@ int fail(nb_sim *s, int code, const char *fmt, ...)
             return fail(sim, NB_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));            \
     } while (0)
 
-// Build threads: each inserts >= 16 bodies; capped so that the cells lost to
-// publication races and the unused rest of each thread's allocation chunk
-// stay a bounded slack on the pool. The cap is what the build's LDS allows in
-// flight (40 KB per 256-thread block: 4 blocks per CU, 1024 on 256 CUs):
-// measured at 4M clustered bodies, 131,072 threads 1.23 ms, 262,144 0.89 ms,
-// 393,216 and 524,288 0.89 ms.
-int build_threads(int n) { return std::min(262144, std::max(256, (n / 16 + 255) / 256 * 256)); }
-
 // Cell pool: the reference's NUMBER_OF_NODES = 2N (java:219-227) scaled by
-// pool_factor / 2, plus 20 cells per build thread for the unused rest of its
-// allocation chunk (at most 15 of 16) and an unpublished spare.
+// pool_factor / 2. The build numbers the tree's cells densely from 0, so the
+// pool holds exactly the cells a tree may have.
 long long pool_cells(const nb_params *p)
 {
-    const double base = std::max(16384.0, std::ceil((double)p->pool_factor * p->n_bodies));
-    return (long long)base + 20LL * build_threads(p->n_bodies);
+    return (long long)std::max(16384.0, std::ceil((double)p->pool_factor * p->n_bodies));
 }
 
 int validate(const nb_params *p)
@@ -311,7 +302,7 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD: e = nb::launch_build(d, s->build_blocks, s->stream); break;
+    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->stream); break;
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
@@ -321,12 +312,7 @@ int launch_stage(nb_sim *s, int stage)
         if (e == hipSuccess && s->head > 0) e = nb::launch_summarize_block(d, 0, s->head, s->stream);
         break;
     }
-    case NB_STAGE_SORT:
-        if (s->head > 0) e = nb::launch_sort_block(d, 0, s->head, s->stream);
-        for (int L = s->head; L < s->split && e == hipSuccess; ++L)
-            e = nb::launch_sort_level(d, L, hint_cells(s, L), s->stream);
-        if (e == hipSuccess) e = nb::launch_sort_block(d, s->split, nb::kMaxDepth + 1, s->stream);
-        break;
+    case NB_STAGE_SORT: e = nb::launch_sort_keys(d, s->stream); break;
     case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:
         if (!fused) e = nb::launch_integrate(d, s->stream);
@@ -492,10 +478,12 @@ void free_all(nb_sim *s)
     nb::Dev &d = s->d;
     if (s->ord) (void)hipStreamSynchronize(s->ord); // it may still be writing d.gorder
     void *ptrs[] = {d.ctl,   d.pm4,   d.vel4,   d.acc4,          d.pos64,    d.slot,
-                    d.cinfo, d.count, d.start,  d.com64,         d.cchild,   d.lvl_list, d.clvl,
+                    d.cinfo, d.count, d.com64,         d.cchild,   d.lvl_list, d.clvl,
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part, d.gcost, d.gorder,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
-                    s->mlpbuf, s->chk_acc, s->vtx};
+                    s->mlpbuf, s->chk_acc, s->vtx, d.kb_key[0], d.kb_key[1], d.kb_val[0], d.kb_val[1],
+                    d.kb_deep, d.kb_hist, d.kb_offs, d.kb_tot, d.kb_ipar, d.kb_lpar, d.kb_dnode, d.kb_lo,
+                    d.kb_own, d.kb_ktop, d.kb_obase, d.kb_scan};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
@@ -753,7 +741,7 @@ int nb_create(const nb_params *p, nb_sim **out)
     if (d.fp64) ok = ok && dalloc(&d.pos64, (size_t)n) == hipSuccess;
     ok = ok && dalloc(&d.slot, 8 * cap) == hipSuccess;
     ok = ok && dalloc(&d.cinfo, cap) == hipSuccess;
-    ok = ok && dalloc(&d.count, cap) == hipSuccess && dalloc(&d.start, cap) == hipSuccess;
+    ok = ok && dalloc(&d.count, cap) == hipSuccess;
     ok = ok && dalloc(&d.com64, cap) == hipSuccess;
     ok = ok && dalloc(&d.cchild, 8 * cap) == hipSuccess;
     ok = ok && dalloc(&d.lvl_list, cap) == hipSuccess && dalloc(&d.clvl, cap) == hipSuccess;
@@ -764,6 +752,18 @@ int nb_create(const nb_params *p, nb_sim **out)
     ok = ok && dalloc(&d.bbox_part, (size_t)6 * nb::kBBoxBlocks) == hipSuccess;
     const size_t ngroups = ((size_t)n + 63) / 64;
     ok = ok && dalloc(&d.gcost, ngroups) == hipSuccess && dalloc(&d.gorder, ngroups) == hipSuccess;
+    {
+        const size_t ktiles = ((size_t)n + nb::kKeyTile - 1) / nb::kKeyTile;
+        const size_t stiles = ((size_t)n + nb::kKeyScanTile - 1) / nb::kKeyScanTile;
+        for (int b = 0; b < 2; ++b)
+            ok = ok && dalloc(&d.kb_key[b], (size_t)n) == hipSuccess && dalloc(&d.kb_val[b], (size_t)n) == hipSuccess;
+        ok = ok && dalloc(&d.kb_deep, 3 * (size_t)n) == hipSuccess;
+        ok = ok && dalloc(&d.kb_hist, 256 * ktiles) == hipSuccess && dalloc(&d.kb_offs, 256 * ktiles) == hipSuccess;
+        ok = ok && dalloc(&d.kb_tot, 256) == hipSuccess;
+        for (int **a : {&d.kb_ipar, &d.kb_lpar, &d.kb_dnode, &d.kb_lo, &d.kb_own, &d.kb_ktop, &d.kb_obase})
+            ok = ok && dalloc(a, (size_t)n) == hipSuccess;
+        ok = ok && dalloc(&d.kb_scan, stiles) == hipSuccess;
+    }
     if (!ok) {
         fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
         return bail(NB_E_HIP);
@@ -781,7 +781,6 @@ int nb_create(const nb_params *p, nb_sim **out)
     if (!ok) return bail(NB_E_HIP);
 
     s->bbox_blocks = std::max(1, std::min(nb::kBBoxBlocks, (n + 2047) / 2048));
-    s->build_blocks = build_threads(n) / 256;
     *out = s;
     return NB_OK;
 }

@@ -9,7 +9,6 @@
 //            cinfo : int4[cap]      (parent, level | octant << 8, epoch, 0), written when
 //                                   the cell is published
 //            count : int32[cap]     bodies in the cell's subtree
-//            start : int32[cap]     first sorted position of the subtree
 //            com64 : double4[cap]   FP64 centre of mass (x, y, z) and mass (w)
 //            cchild: int32[8*cap]   children compacted to the front; cells as
 //                                   N + cell, bodies by index (-1 after the last)
@@ -28,6 +27,15 @@
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
 //            sorted : int32[N]      body index at each sorted position
+//   build    kb_key, kb_val : uint64[N] and int32[N], two of each: octant-path
+//                                   keys and body indices through the radix
+//                                   passes, the sorted pair in [0]
+//            kb_deep: uint64[3N]    levels 21..83 of the bodies of equal-key runs
+//            binary radix tree, int32[N] each: kb_ipar, kb_lpar (parents of
+//                                   internal nodes and of sorted bodies),
+//                                   kb_dnode (common prefix in bits), kb_lo
+//                                   (first sorted body), kb_own / kb_ktop /
+//                                   kb_obase (octree levels and first cell held)
 //
 // Body records are 16-byte AoS: streaming kernels still read them with fully
 // coalesced 16-byte lanes, and the gathers of the tree stages touch one cache
@@ -35,8 +43,9 @@
 //
 // Node numbering follows the reference (buildtree.cl:277, sort.cl:223): a
 // child value v < 0 is empty, v < N is a body, v >= N is cell v - N. Cell 0 is
-// the root. Cells are taken from the pool upward from 1 (the reference takes
-// them downward from NN, buildtree.cl:109); cell numbers carry no order.
+// the root. Cells are numbered upward from 1 in octant-path order of their
+// bodies (the reference takes them downward from NN, buildtree.cl:109); cell
+// numbers carry no meaning beyond that.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,7 +57,7 @@ constexpr int kMaxDepth = 64;        // reference MAXDEPTH, calculateforce.cl:12
 constexpr int kBuildLevelLimit = 64; // cells live on levels < 64, bodies at depth <= 64
 constexpr int kBBoxBlocks = 1024;    // partial-result slots of the bounding box
 constexpr int kLevelBlocks = 512;    // blocks of the level histogram / scatter
-constexpr int kTailThreads = 512;    // the single block of the deepest summarize / sort levels
+constexpr int kTailThreads = 512;    // the single block of the deepest summarize levels
 constexpr unsigned char kNotLive = 0xff; // clvl of a cell that is not in the tree
 constexpr int kCostKeys = 32;        // cost classes of the coarse traversal's launch order
 constexpr uint32_t kEmpty = 0xffffffffu;
@@ -94,7 +103,7 @@ struct Dev {
     double4 *pos64;
     unsigned long long *slot;
     int4 *cinfo;
-    int *count, *start;
+    int *count;
     double4 *com64;
     int *cchild;
     float4 *cdata;
@@ -111,11 +120,26 @@ struct Dev {
     float *bbox_part; // 6 * kBBoxBlocks
     int *gcost;       // per 64-body group: cost of its last coarse traversal (children tested)
     int *gorder;      // the groups, most expensive first (the coarse traversal's launch order)
+    // the key-sorted build (nb_keys.hip): octant-path keys and bodies (two
+    // buffers each for the radix passes; the sorted result ends in [0]), the
+    // deep key words of runs (3 * n), the radix sort's per-tile digit counts
+    // and offsets (256 * tiles) and digit totals (256), and the binary radix
+    // tree over the sorted keys (n each)
+    unsigned long long *kb_key[2];
+    int *kb_val[2];
+    unsigned long long *kb_deep;
+    unsigned *kb_hist, *kb_offs, *kb_tot;
+    int *kb_ipar, *kb_lpar, *kb_dnode, *kb_lo, *kb_own, *kb_ktop, *kb_obase, *kb_scan;
 };
 
 // ---- launchers (nb_tree.hip, nb_force.hip) ---------------------------------
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
-hipError_t launch_build(const Dev &d, int blocks, hipStream_t st);
+// the build from sorted octant-path keys (the same child slots and cell records)
+hipError_t launch_build_keys(const Dev &d, hipStream_t st);
+// stage 4 after the key-sorted build: sorted[] and bnode[] from its body order
+hipError_t launch_sort_keys(const Dev &d, hipStream_t st);
+constexpr int kKeyTile = 2048; // keys per radix-sort tile
+constexpr int kKeyScanTile = 1024;
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
 // Summarize and sort read each level's cell count on the device; the host's
 // cells_hint (the level's size in an earlier tree) only sizes the grid. The
@@ -124,8 +148,6 @@ hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by l
 // the top ones, where the levels hold few cells.
 hipError_t launch_summarize_level(const Dev &d, int level, int cells_hint, hipStream_t st);
 hipError_t launch_summarize_block(const Dev &d, int lo, int hi, hipStream_t st);
-hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st);
-hipError_t launch_sort_block(const Dev &d, int lo, int hi, hipStream_t st);
 // mode: 0 = store accelerations (reference semantics), 1 = also integrate
 // (fused kick-drift-kick), 2 = probe: write accelerations to d.probe only
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st);

@@ -1,15 +1,13 @@
-// nb_tree.hip -- bounding box, octree build, summarize, sort, integrate and
-// the vertex export, for gfx950.
+// nb_tree.hip -- bounding box, summarize, integrate and the vertex export,
+// for gfx950 (the build and the sort are in nb_keys.hip).
 //
-// Only the build communicates between workgroups inside a kernel, through
-// agent-scope atomics on the child slots and the cell counter. Summarize and
-// sort run one launch per tree level, so every value they read was written by
-// an earlier launch: no fences, no waiting on other threads; the deepest
-// levels, which hold few cells, share one single-block launch with a block
-// barrier between levels. (On MI355X each XCD has its own L2; an agent-scope
-// acquire/release costs an L2 invalidate / write-back, which is why the
-// per-cell hand-offs of the reference's summarize and sort are replaced by
-// level launches.)
+// No kernel here waits on another workgroup. Summarize runs one launch per
+// tree level, so every value it reads was written by an earlier launch: no
+// fences, no waiting on other threads; the deepest levels, which hold few
+// cells, share one single-block launch with a block barrier between levels.
+// (On MI355X each XCD has its own L2; an agent-scope acquire/release costs an
+// L2 invalidate / write-back, which is why the per-cell hand-offs of the
+// reference's summarize are replaced by level launches.)
 #include "nb_internal.h"
 
 namespace nb {
@@ -19,11 +17,6 @@ namespace nb {
 __device__ __forceinline__ int dec_slot(unsigned long long w, uint32_t ep)
 {
     return ((uint32_t)(w >> 32) == ep) ? (int)(uint32_t)w : -1;
-}
-
-__device__ __forceinline__ unsigned long long tag(uint32_t ep, int v)
-{
-    return ((unsigned long long)ep << 32) | (uint32_t)v;
 }
 
 // Octant of a body relative to a cell centre: bit0 x, bit1 y, bit2 z, set when
@@ -40,10 +33,8 @@ __device__ __forceinline__ void set_error(Ctl *ctl, int code)
                                          __ATOMIC_RELAXED, AGENT);
 }
 
-// A cell is in the tree when it was published in this tree's epoch: a
-// published cell is never unlinked (a slot that holds a cell never changes
-// within a step), and its record is written right after the publishing CAS.
-// Cells a thread took from the pool but never published keep an older epoch.
+// A cell is in the tree when its record carries this tree's epoch: the build
+// writes cells 0 .. n_cells-1, and records past them keep an older epoch.
 __device__ __forceinline__ bool cell_live(const Dev &d, int c, uint32_t ep, int *level)
 {
     if (c == 0) {
@@ -173,212 +164,6 @@ __global__ __launch_bounds__(256) void k_bbox_final(Dev d, int blocks)
         c->epoch += 1;  // every child slot of the last tree turns stale
         c->step += 1;   // boundingbox.cl:195
     }
-}
-
-// ---------------------------------------------------------------------------
-// Stage 2: octree build (buildtree.cl:12-200), lock-free.
-//
-// A child slot is a 64-bit word (epoch << 32 | child); a word of another
-// epoch reads as empty, so the previous step's tree needs no clearing. Slot
-// states only move forward within a step: empty -> body -> cell, and every
-// change is a CAS from the value the thread read to the new one:
-//   * a body goes into an empty slot with one CAS;
-//   * a slot that holds a body d is split by publishing a fresh cell c in it
-//     (CAS d -> N+c); d is now out of the tree and the thread re-inserts it,
-//     starting at c, before it goes on with its own body (a small per-thread
-//     stack in LDS keeps the bodies still to be placed). c is published with
-//     no child yet, so no other write has to become visible before it: no
-//     release fence is needed (on MI355X one costs an L2 write-back).
-// A lost race leaves c unpublished and untouched; the thread keeps it as a
-// spare for its next split, so at most one cell per thread is ever unused.
-// Cell centres are never read from memory: each thread derives them from the
-// root along its own path with the reference's arithmetic
-// (buildtree.cl:124-136), which is what makes the tree bit-identical.
-//
-// Because a CAS compares with the current value and a slot holding a cell
-// never changes again, a stale read costs at most a failed CAS: descents read
-// slots with ordinary cached loads, and each thread keeps the cells of its
-// previous path in LDS so that the next body of its chunk (its octant-order
-// neighbour) skips the levels the two paths share.
-//
-// Insertion order: thread t inserts a contiguous chunk of the previous step's
-// sorted (octant-path) order, so consecutive insertions of a thread share most
-// of their path and concurrent threads work in different regions. A first
-// launch with a single block inserts an evenly spaced sample of that order,
-// so the top levels exist before all threads start.
-// ---------------------------------------------------------------------------
-constexpr int kPathCache = 24; // levels of the previous path kept per thread
-constexpr int kPending = 8;    // bodies waiting to be re-inserted, per thread
-
-__device__ __forceinline__ int path_oct(unsigned long long w0, unsigned long long w1,
-                                        unsigned long long w2, int level)
-{
-    const unsigned long long w = level < 21 ? w0 : (level < 42 ? w1 : w2);
-    return (int)((w >> (3 * (level % 21))) & 7ull);
-}
-
-__device__ __forceinline__ void set_path_oct(unsigned long long &w0, unsigned long long &w1,
-                                             unsigned long long &w2, int level, int oct)
-{
-    const int sh = 3 * (level % 21);
-    const unsigned long long clr = ~(7ull << sh), val = (unsigned long long)oct << sh;
-    if (level < 21) w0 = (w0 & clr) | val;
-    else if (level < 42) w1 = (w1 & clr) | val;
-    else w2 = (w2 & clr) | val;
-}
-
-// A thread takes cells from the pool kAllocChunk at a time (one atomic per
-// chunk). Unused cells of a chunk stay dead: they are never linked, so the
-// level lists never see them; the pool carries the slack.
-constexpr int kAllocChunk = 16;
-
-// seed: insert sorted[k * stride] for k < nseed; otherwise every other body,
-// a contiguous chunk of the sorted order per thread
-__global__ __launch_bounds__(256) void k_build(Dev d, int stride, int nseed, int seed)
-{
-    __shared__ int pc[kPathCache][256];
-    __shared__ int pend_body[kPending][256];
-    __shared__ unsigned pend_cell[kPending][256]; // cell << 6 | level
-    const int n = d.n, cap = d.cap;
-    Ctl *ctl = d.ctl;
-    if (ctl->error) return;
-    const uint32_t ep = ctl->epoch;
-    const float radius = ctl->radius;
-    const float rx = ctl->root[0], ry = ctl->root[1], rz = ctl->root[2];
-    const int tid = threadIdx.x;
-    const int gtid = blockIdx.x * blockDim.x + tid;
-    const int T = gridDim.x * blockDim.x;
-    int first, last, step;
-    if (seed) {
-        first = gtid;
-        last = nseed;
-        step = T;
-    } else {
-        const int chunk = (n + T - 1) / T;
-        first = gtid * chunk;
-        last = min(n, first + chunk);
-        step = 1;
-    }
-    int my_max = 0;
-    int spare = -1;
-    int pool_next = 0, pool_end = 0; // this thread's chunk of the cell pool
-    bool failed = false;
-    unsigned long long po0 = 0, po1 = 0, po2 = 0; // octants of the cached path
-    int plen = 0; // pc[1 .. plen-1] hold the cells of the cached path
-    for (int i = first; i < last && !failed; i += step) {
-        if (!seed && i % stride == 0 && i / stride < nseed) continue; // placed by the seed launch
-        const int si = seed ? i * stride : i;
-        const int b = d.sorted[si];
-        int cur = b, npend = 0;
-        float4 me = d.pm4[cur];
-        int p = 0, level = 0;
-        float cx = rx, cy = ry, cz = rz, r = radius;
-        // follow the previous path while the octants agree
-        while (level + 1 < plen) {
-            const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
-            if (oct != path_oct(po0, po1, po2, level)) break;
-            const float rh = r * 0.5f;
-            cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
-            cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-            cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
-            r = rh;
-            p = pc[level + 1][tid];
-            ++level;
-        }
-        plen = level + 1;
-        for (;;) { // place `cur`, starting in cell p at `level`
-            bool placed = false;
-            for (;;) {
-                const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
-                unsigned long long *sp = d.slot + (size_t)p * 8 + oct;
-                unsigned long long w = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                int v = dec_slot(w, ep);
-                int next = -1; // cell to descend into
-                for (int tries = 0;; ++tries) {
-                    if (tries > 16) { set_error(ctl, kErrSync); failed = true; break; }
-                    if (v >= n) { next = v - n; break; }
-                    if (v < 0) {
-                        if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, cur), __ATOMIC_RELAXED,
-                                                                 __ATOMIC_RELAXED, AGENT)) {
-                            placed = true;
-                            break;
-                        }
-                        v = dec_slot(w, ep);
-                        continue;
-                    }
-                    // slot holds body v: split it (buildtree.cl:102-181)
-                    if (level + 1 >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
-                    int c = spare;
-                    if (c < 0) {
-                        if (pool_next == pool_end) {
-                            pool_next = __hip_atomic_fetch_add(&ctl->n_cells, kAllocChunk, __ATOMIC_RELAXED, AGENT);
-                            pool_end = pool_next + kAllocChunk;
-                        }
-                        c = pool_next++;
-                    }
-                    spare = -1;
-                    if (c >= cap) { set_error(ctl, kErrPool); failed = true; break; }
-                    if (__hip_atomic_compare_exchange_strong(sp, &w, tag(ep, n + c), __ATOMIC_RELAXED,
-                                                             __ATOMIC_RELAXED, AGENT)) {
-                        d.cinfo[c] = make_int4(p, (level + 1) | (oct << 8), (int)ep, 0);
-                        // cur goes on later from c; the displaced body v first
-                        if (npend >= kPending) { set_error(ctl, kErrSync); failed = true; break; }
-                        pend_body[npend][tid] = cur;
-                        pend_cell[npend][tid] = ((unsigned)c << 6) | (unsigned)(level + 1);
-                        ++npend;
-                        cur = v;
-                        me = d.pm4[cur];
-                        next = c;
-                        break;
-                    }
-                    spare = c;
-                    v = dec_slot(w, ep);
-                }
-                if (failed || placed) {
-                    if (placed) my_max = max(my_max, level + 1);
-                    break;
-                }
-                if (next >= cap) { set_error(ctl, kErrSync); failed = true; break; }
-                // descend (buildtree.cl:277-289), remembering the path
-                set_path_oct(po0, po1, po2, level, oct);
-                if (level + 1 < kPathCache) {
-                    pc[level + 1][tid] = next;
-                    plen = level + 2;
-                }
-                const float rh = r * 0.5f;
-                cx = (cx - rh) + ((oct & 1) ? r : 0.0f); // buildtree.cl:124-136
-                cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-                cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
-                r = rh;
-                p = next;
-                ++level;
-                if (level >= kBuildLevelLimit) { set_error(ctl, kErrDepth); failed = true; break; }
-            }
-            if (failed || npend == 0) break;
-            // next waiting body: it continues in the cell where it was displaced
-            --npend;
-            cur = pend_body[npend][tid];
-            const unsigned e = pend_cell[npend][tid];
-            p = (int)(e >> 6);
-            const int target = (int)(e & 63u);
-            me = d.pm4[cur];
-            cx = rx; cy = ry; cz = rz; r = radius;
-            for (level = 0; level < target; ++level) {
-                const int oct = octant(cx, cy, cz, me.x, me.y, me.z);
-                const float rh = r * 0.5f;
-                cx = (cx - rh) + ((oct & 1) ? r : 0.0f);
-                cy = (cy - rh) + ((oct & 2) ? r : 0.0f);
-                cz = (cz - rh) + ((oct & 4) ? r : 0.0f);
-                r = rh;
-            }
-            plen = min(plen, level + 1);
-        }
-    }
-    // running maximum depth (buildtree.cl:199)
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, __shfl_xor(my_max, o));
-    if ((threadIdx.x & 63) == 0 && my_max > 0)
-        __hip_atomic_fetch_max(&ctl->max_depth, my_max, __ATOMIC_RELAXED, AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -674,74 +459,6 @@ __global__ __launch_bounds__(kTailThreads) void k_summarize_block(Dev d, int lo,
 }
 
 // ---------------------------------------------------------------------------
-// Stage 4: sort (sort.cl:12-76), one launch per level from the root down. A
-// cell's start arrived from its parent in the previous launch; it hands
-// consecutive offsets to its children in octant order: child cells get their
-// start, bodies their sorted position and their (x, y, z, m) in sorted order.
-// The result is the octant-path (Morton) order of the bodies.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void sort_cell(const Dev &d, int level, int first, int i, int k, int cells)
-{
-    // eight lanes per cell, one per compacted child; each lane's offset is the
-    // exclusive prefix sum of its siblings' sizes (a body counts 1)
-    const bool live = i < cells; // whole 8-lane groups; every lane joins the shuffles
-    const int n = d.n;
-    Ctl *ctl = d.ctl;
-    int c = 0, v = -1, base = 0;
-    if (live) {
-        c = d.lvl_list[first + i];
-        base = level == 0 ? 0 : d.start[c]; // root: boundingbox.cl:189
-        v = d.cchild[(size_t)c * 8 + k];
-    }
-    const int size = v >= n ? d.count[v - n] : (v >= 0 ? 1 : 0);
-    int incl = size; // inclusive prefix over the 8 lanes of this cell
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        const int u = __shfl_up(incl, o, 8);
-        if (k >= o) incl += u;
-    }
-    const int st = base + incl - size;
-    if (!live) return;
-    if (level == 0 && k == 0) d.start[c] = 0;
-    if (v >= n) { // sort.cl:223-237
-        d.start[v - n] = st;
-    } else if (v >= 0) { // sort.cl:238-245
-        if (st < 0 || st >= n) {
-            set_error(ctl, kErrSync);
-            return;
-        }
-        d.sorted[st] = v;
-        d.bnode[st] = d.cdata[(size_t)c * 8 + k];
-    }
-}
-
-// One level, cell count read on the device, grid-stride (as k_summarize_level).
-__global__ __launch_bounds__(256) void k_sort_level(Dev d, int level)
-{
-    const Ctl *ctl = d.ctl;
-    if (level >= ctl->levels) return;
-    const int first = ctl->lvl_start[level], cells = ctl->lvl_start[level + 1] - first;
-    const int k = threadIdx.x & 7;
-    for (int i0 = blockIdx.x * (blockDim.x >> 3); i0 < cells; i0 += gridDim.x * (blockDim.x >> 3))
-        sort_cell(d, level, first, i0 + (threadIdx.x >> 3), k, cells);
-}
-
-// Levels lo .. hi-1 (those the tree has), top-down, in one block (as
-// k_summarize_block).
-__global__ __launch_bounds__(kTailThreads) void k_sort_block(Dev d, int lo, int hi)
-{
-    const Ctl *ctl = d.ctl;
-    const int end = min(hi, ctl->levels);
-    const int k = threadIdx.x & 7;
-    for (int L = lo; L < end; ++L) {
-        const int first = ctl->lvl_start[L], cells = ctl->lvl_start[L + 1] - first;
-        for (int i0 = 0; i0 < cells; i0 += kTailThreads / 8)
-            sort_cell(d, L, first, i0 + (threadIdx.x >> 3), k, cells);
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Stage 6: integrate (integrate.cl:17-70): kick with the acceleration just
 // computed, drift, kick. FP64 positions when NB_FP64_POSITIONS is set.
 // ---------------------------------------------------------------------------
@@ -828,17 +545,6 @@ hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st)
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, d, blocks);
     return hipGetLastError();
 }
-hipError_t launch_build(const Dev &d, int blocks, hipStream_t st)
-{
-    // seed the top levels with 1024 evenly spaced bodies (one per thread), then
-    // every other body. Measured at 4M clustered bodies: no seed 4.0 ms, 4096
-    // seeds on one block 1.9 ms, 1024 seeds on four blocks 1.2 ms.
-    const int nseed = d.n >= 65536 ? 1024 : 0;
-    const int stride = nseed ? d.n / nseed : 1;
-    if (nseed) hipLaunchKernelGGL(k_build, dim3(nseed / 256), dim3(256), 0, st, d, stride, nseed, 1);
-    hipLaunchKernelGGL(k_build, dim3(blocks), dim3(256), 0, st, d, stride, nseed, 0);
-    return hipGetLastError();
-}
 hipError_t launch_levels(const Dev &d, hipStream_t st)
 {
     hipLaunchKernelGGL(k_level_hist, dim3(kLevelBlocks), dim3(256), 0, st, d);
@@ -863,16 +569,6 @@ hipError_t launch_summarize_block(const Dev &d, int lo, int hi, hipStream_t st)
         hipLaunchKernelGGL(k_summarize_block<true>, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
     else
         hipLaunchKernelGGL(k_summarize_block<false>, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
-    return hipGetLastError();
-}
-hipError_t launch_sort_level(const Dev &d, int level, int cells_hint, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_sort_level, level_grid(cells_hint), dim3(256), 0, st, d, level);
-    return hipGetLastError();
-}
-hipError_t launch_sort_block(const Dev &d, int lo, int hi, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_sort_block, dim3(1), dim3(kTailThreads), 0, st, d, lo, hi);
     return hipGetLastError();
 }
 hipError_t launch_integrate(const Dev &d, hipStream_t st)

@@ -53,3 +53,31 @@ def cell_multiset(count, x, y, z, m):
                                             ("m", "<f4")])
     rec["count"], rec["x"], rec["y"], rec["z"], rec["m"] = count[keep], x[keep], y[keep], z[keep], m[keep]
     return np.sort(rec, order=["count", "x", "y", "z", "m"])
+
+
+def tight_clump(n=8192, seed=9, target=(0.3, -0.2, 0.1)):
+    """A uniform ball of n bodies plus 27 bodies on a 3x3x3 grid inside one
+    cell of tree level 21 (the cell around `target`, from the root the bounding
+    box gives, with the build's FP32 centre recurrence), a quarter of the cell's
+    half size apart: the 27 share their first 21 octants and separate a few
+    levels deeper."""
+    from nbody_amd import universe as U
+
+    f = np.float32
+    st = [np.array(a, copy=True) for a in U.uniform_ball(n, seed=seed)]
+    c = [f(0.5) * (a.min() + a.max()) for a in st[:3]]
+    r = f(0.5) * max(a.max() - a.min() for a in st[:3])
+    t = [f(v) for v in target]
+    for _ in range(21):
+        rh = r * f(0.5)
+        c = [(ck - rh) + (r if ck < tk else f(0.0)) for ck, tk in zip(c, t)]
+        r = rh
+    step = r * f(0.25)
+    g = np.array([-1, 0, 1], np.float32) * step
+    gx, gy, gz = np.meshgrid(g, g, g, indexing="ij")
+    pos = [np.float32(ck) + gk.ravel() for ck, gk in zip(c, (gx, gy, gz))]
+    m = np.float32(1.0) / np.float32(n + 27)
+    out = [np.concatenate([st[k], pos[k]]) for k in range(3)]
+    out += [np.concatenate([st[k], np.zeros(27, np.float32)]) for k in range(3, 6)]
+    out.append(np.full(n + 27, m, np.float32))
+    return tuple(np.ascontiguousarray(a, np.float32) for a in out)

@@ -16,7 +16,7 @@ Bars (bit-exact where the arithmetic is fixed, stated tolerance otherwise):
 import numpy as np
 import pytest
 
-from helpers import cell_multiset, make_oracle, make_sim, rel_err
+from helpers import cell_multiset, make_oracle, make_sim, rel_err, tight_clump
 
 pytestmark = pytest.mark.gpu
 
@@ -32,10 +32,12 @@ def _universes():
         "clusters_32k": lambda: U.gaussian_clusters(32768, seed=4),
         "ragged_1000": lambda: U.uniform_ball(1000, seed=7),
         "ragged_12345": lambda: U.plummer(12345, seed=8),
+        # 27 bodies sharing their first 21 octants (helpers.tight_clump)
+        "tight_run_8219": lambda: tight_clump(),
     }
 
 
-CASES = ["uniform_ball_16k", "plummer_32k", "clusters_32k", "ragged_1000", "ragged_12345"]
+CASES = ["uniform_ball_16k", "plummer_32k", "clusters_32k", "ragged_1000", "ragged_12345", "tight_run_8219"]
 PHYS = dict(theta=0.5, softening=0.05, dt=1e-3)
 
 

@@ -434,3 +434,21 @@ def test_cpu_baseline_fp32_force_is_the_same_traversal(oracle, vote):
     mask = np.ones(len(st[0]), bool)
     mask[rows] = False
     assert not a2[mask].any() and 0 < ni2 < ni
+
+
+def test_tight_clump_premise():
+    """The parity case tight_run_8219 (helpers.tight_clump) places 27 distinct
+    bodies in one cell of level 21: they share their first 21 octants (one
+    63-bit key in the key-sorted build, more than its 16-body register path)
+    and every other body leaves that cell's path earlier."""
+    from helpers import tight_clump
+
+    st = tight_clump()
+    pts = np.stack(st[:3], 1)
+    assert len(np.unique(pts[-27:], axis=0)) == 27
+    orc = make_oracle(st, 0.5, 0.05, 1e-3, 16)
+    assert orc.run_stages(1, 2) == 0
+    p, d = orc.paths()
+    assert np.all(d[-27:] > 21)
+    shared = (p[:, 0] == p[-1, 0]) & (d > 21)
+    assert shared.sum() == 27 and np.all(shared[-27:])
