@@ -73,6 +73,7 @@ struct nb_sim {
     hipStream_t ord = nullptr;
     hipEvent_t ord_in = nullptr, ord_out = nullptr;
     bool ord_ready = false; // d.gorder holds an order once ord_out has passed
+    bool keys_warm = false; // d.sorted holds the last tree's body order (the build starts from it)
     // copyVertices staging: 2 N float4 (positions, then velocities), allocated
     // by the first nb_copy_vertices call and kept until nb_destroy
     float4 *vtx = nullptr;
@@ -302,7 +303,7 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->stream); break;
+    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->keys_warm, s->stream); break;
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
@@ -312,7 +313,10 @@ int launch_stage(nb_sim *s, int stage)
         if (e == hipSuccess && s->head > 0) e = nb::launch_summarize_block(d, 0, s->head, s->stream);
         break;
     }
-    case NB_STAGE_SORT: e = nb::launch_sort_keys(d, s->stream); break;
+    case NB_STAGE_SORT:
+        e = nb::launch_sort_keys(d, s->stream);
+        s->keys_warm = true; // sorted[] holds this tree's body order for the next build
+        break;
     case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:
         if (!fused) e = nb::launch_integrate(d, s->stream);
@@ -482,8 +486,8 @@ void free_all(nb_sim *s)
                     d.lvl_blockhist,  d.cdata,  d.bnode,  d.sorted, d.probe, d.bbox_part, d.gcost, d.gorder,
                     s->chk_idx, s->chk_part, s->chk_out, d.cq64, d.cqa, d.cqb, d.cmac,
                     s->mlpbuf, s->chk_acc, s->vtx, d.kb_key[0], d.kb_key[1], d.kb_val[0], d.kb_val[1],
-                    d.kb_deep, d.kb_hist, d.kb_offs, d.kb_tot, d.kb_ipar, d.kb_lpar, d.kb_dnode, d.kb_lo,
-                    d.kb_own, d.kb_ktop, d.kb_obase, d.kb_scan};
+                    d.kb_deep, d.kb_hist, d.kb_offs, d.kb_tot, d.kb_ipar, d.kb_lpar, d.kb_lo,
+                    d.kb_own, d.kb_ktop, d.kb_obase, d.kb_scan, d.kb_tdep, d.kb_spl, d.kb_bcnt, d.kb_bfill, d.kb_bstart, d.kb_big};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : s->tev)
@@ -760,9 +764,13 @@ int nb_create(const nb_params *p, nb_sim **out)
         ok = ok && dalloc(&d.kb_deep, 3 * (size_t)n) == hipSuccess;
         ok = ok && dalloc(&d.kb_hist, 256 * ktiles) == hipSuccess && dalloc(&d.kb_offs, 256 * ktiles) == hipSuccess;
         ok = ok && dalloc(&d.kb_tot, 256) == hipSuccess;
-        for (int **a : {&d.kb_ipar, &d.kb_lpar, &d.kb_dnode, &d.kb_lo, &d.kb_own, &d.kb_ktop, &d.kb_obase})
+        for (int **a : {&d.kb_ipar, &d.kb_lpar, &d.kb_lo, &d.kb_own, &d.kb_ktop, &d.kb_obase})
             ok = ok && dalloc(a, (size_t)n) == hipSuccess;
-        ok = ok && dalloc(&d.kb_scan, stiles) == hipSuccess;
+        ok = ok && dalloc(&d.kb_scan, stiles) == hipSuccess && dalloc(&d.kb_tdep, stiles) == hipSuccess;
+        const size_t S = (size_t)nb::bucket_samples(n);
+        ok = ok && dalloc(&d.kb_spl, S) == hipSuccess && dalloc(&d.kb_bcnt, S + 1) == hipSuccess &&
+             dalloc(&d.kb_bfill, S + 1) == hipSuccess && dalloc(&d.kb_bstart, S + 2) == hipSuccess &&
+             dalloc(&d.kb_big, S + 2) == hipSuccess;
     }
     if (!ok) {
         fail(s, NB_E_HIP, "device allocation failed for %d bodies / %d cells", n, d.cap);
@@ -822,8 +830,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
         r = upload(s, d.pos64, p64.data(), n * sizeof(double4));
         if (r) return r;
     }
-    // the build inserts bodies in the previous step's sorted order; before the
-    // first step that order is the identity
+    // sorted[] before the first tree: the identity
     {
         std::vector<int> ident((size_t)d.n);
         for (int i = 0; i < d.n; ++i) ident[i] = i;
@@ -841,6 +848,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     s->cell_inter_before = 0;
     s->hint_valid = false; // the first step reads its level sizes back
     s->ord_ready = false;  // the launch order of the coarse traversal belongs to the old bodies
+    s->keys_warm = false;  // and so does sorted[]: the next build sorts from scratch
     reset_control(s);
     s->have_state = true;
     s->err.clear();
@@ -1644,6 +1652,7 @@ int nb_get_counters(nb_sim *s, nb_counters *out)
     out->cells = c.n_cells;
     out->max_depth = c.max_depth;
     out->live_cells = c.live_cells;
+    out->large_buckets = c.large_buckets;
     out->interactions = s->last_interactions;
     out->cell_interactions = s->last_cell_interactions;
     out->error = c.error;

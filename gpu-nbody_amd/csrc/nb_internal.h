@@ -32,10 +32,11 @@
 //                                   passes, the sorted pair in [0]
 //            kb_deep: uint64[3N]    levels 21..83 of the bodies of equal-key runs
 //            binary radix tree, int32[N] each: kb_ipar, kb_lpar (parents of
-//                                   internal nodes and of sorted bodies),
-//                                   kb_dnode (common prefix in bits), kb_lo
+//                                   internal nodes and of sorted bodies), kb_lo
 //                                   (first sorted body), kb_own / kb_ktop /
 //                                   kb_obase (octree levels and first cell held)
+//            kb_scan, kb_tdep: per scan tile of the cell counts, its sum and depth
+//            kb_spl, kb_bcnt, kb_bfill, kb_bstart: the buckets of a warm build
 //
 // Body records are 16-byte AoS: streaming kernels still read them with fully
 // coalesced 16-byte lanes, and the gathers of the tree stages touch one cache
@@ -88,6 +89,7 @@ struct Ctl {
     int live_cells;      // cells linked into the tree by the last build
     unsigned long long cell_interactions; // cumulative cell interactions of the coarse traversal
     int lvl_start[kMaxDepth + 2]; // first lvl_list entry of each level
+    int large_buckets;   // buckets of warm builds sorted through device memory, cumulative
 };
 
 struct Dev {
@@ -129,17 +131,25 @@ struct Dev {
     int *kb_val[2];
     unsigned long long *kb_deep;
     unsigned *kb_hist, *kb_offs, *kb_tot;
-    int *kb_ipar, *kb_lpar, *kb_dnode, *kb_lo, *kb_own, *kb_ktop, *kb_obase, *kb_scan;
+    int *kb_ipar, *kb_lpar, *kb_lo, *kb_own, *kb_ktop, *kb_obase, *kb_scan, *kb_tdep;
+    // warm builds: splitters (S = bucket_samples(n)), bucket counts, fill
+    // counters (S + 1 each) and starts (S + 2)
+    unsigned long long *kb_spl;
+    unsigned *kb_bcnt, *kb_bfill, *kb_bstart;
+    int *kb_big; // S + 2: the count of buckets above kBucketCap, then their numbers
 };
 
 // ---- launchers (nb_tree.hip, nb_force.hip) ---------------------------------
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
 // the build from sorted octant-path keys (the same child slots and cell records)
-hipError_t launch_build_keys(const Dev &d, hipStream_t st);
+hipError_t launch_build_keys(const Dev &d, bool warm, hipStream_t st);
 // stage 4 after the key-sorted build: sorted[] and bnode[] from its body order
 hipError_t launch_sort_keys(const Dev &d, hipStream_t st);
-constexpr int kKeyTile = 2048; // keys per radix-sort tile
-constexpr int kKeyScanTile = 1024;
+constexpr int kKeyTile = 2048;     // keys per radix-sort tile (first build)
+constexpr int kKeyScanTile = 1024; // cell counts per scan tile
+constexpr int kBucketT = 512;      // warm builds: one splitter per kBucketT bodies
+constexpr int kBucketCap = 1024;   // buckets sorted by one wave up to this size
+inline int bucket_samples(int n) { return (n + kBucketT - 1) / kBucketT; }
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
 // Summarize and sort read each level's cell count on the device; the host's
 // cells_hint (the level's size in an earlier tree) only sizes the grid. The

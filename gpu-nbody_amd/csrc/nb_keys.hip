@@ -65,14 +65,16 @@ __device__ __forceinline__ void key_descend(float &cx, float &cy, float &cz, flo
 }
 
 // ---------------------------------------------------------------------------
-// Keys: 21 levels of each body's octant path, level 0 in bits 62..60.
+// Keys: 21 levels of each body's octant path, level 0 in bits 62..60, for the
+// bodies in the order `order` (the identity when null).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_keys(Dev d, unsigned long long *key, int *val)
+__global__ __launch_bounds__(256) void k_keys(Dev d, const int *__restrict__ order, unsigned long long *key, int *val)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const Ctl *ctl = d.ctl;
     if (i >= d.n || ctl->error) return;
-    const float4 p = d.pm4[i];
+    const int b = order ? order[i] : i;
+    const float4 p = d.pm4[b];
     float cx = ctl->root[0], cy = ctl->root[1], cz = ctl->root[2], r = ctl->radius;
     unsigned long long k = 0;
 #pragma unroll
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_keys(Dev d, unsigned long long *key, in
         key_descend(cx, cy, cz, r, o);
     }
     key[i] = k;
-    val[i] = i;
+    val[i] = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -243,6 +245,500 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const unsigned long l
 }
 
 // ---------------------------------------------------------------------------
+// Warm sort. After the first step the bodies come in the last tree's order
+// (sorted[]), which the new keys nearly keep, and the last tree's sorted keys
+// are still in kb_key[0]. Bucketing by splitters taken from those keys then
+// moves most bodies a short way and every bucket is small:
+//   * splitters: every kBucketT-th of the last tree's sorted keys;
+//   * a body's bucket: the number of splitters <= its key. A block's bodies
+//     are consecutive in the last order, so their buckets are nearly all in
+//     a window of kWin around the one their places suggest: searched in an
+//     LDS copy of the window's splitters, counted in LDS, one atomic per
+//     bucket and block (k_keys_warm, with the keys); the counts' exclusive
+//     scan (k_bucket_scan); then each body takes the next place of its
+//     bucket the same way (k_bucket_scatter);
+//   * each bucket is sorted by one wave (k_bucket_sort): up to kBucketCap
+//     bodies held in registers, stable radix passes over the key bits that
+//     vary inside it, permuted through LDS; a larger bucket (stale order) is
+//     listed and sorted by a block through device memory (k_bucket_sort_big).
+// The order inside a bucket before its sort depends on the atomics; the keys
+// are distinct after the run ordering below (equal keys are ordered by their
+// deeper levels), so the result does not.
+// ---------------------------------------------------------------------------
+constexpr int kBsThreads = 512; // blocks of the large-bucket sort
+constexpr int kBsWaves = kBsThreads / 64;
+static_assert(kBucketCap % 64 == 0, "whole rows of a wave");
+
+// Splitters spl[0 .. S-1]: the keys at positions j * kBucketT of `key` (the
+// last tree's sorted keys, so non-decreasing: a decrease means the buffer does
+// not hold them and stops the build with an error); the bucket counters and
+// the list of large buckets cleared.
+__global__ __launch_bounds__(256) void k_splitters(Ctl *ctl, const unsigned long long *__restrict__ key, int S,
+                                                   unsigned long long *__restrict__ spl, unsigned *__restrict__ bcnt,
+                                                   unsigned *__restrict__ bfill, int *__restrict__ big)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctl->error || j > S) return;
+    if (j < S) {
+        const unsigned long long k = key[(size_t)j * kBucketT];
+        spl[j] = k;
+        if (j > 0 && key[(size_t)(j - 1) * kBucketT] > k) flag_error(ctl, kErrSync);
+    }
+    bcnt[j] = 0;
+    bfill[j] = 0;
+    if (j == 0) big[0] = 0;
+}
+
+// the first j with spl[j] > k (S if none): spl is non-decreasing; searched
+// outward from g, then by bisection (global memory). Invariant: (lo < 0 or spl[lo] <= k) and
+// (hi >= S or spl[hi] > k).
+__device__ __forceinline__ int bucket_of(const unsigned long long *__restrict__ spl, int S, unsigned long long k, int g)
+{
+    g = min(max(g, 0), S);
+    int lo, hi;
+    if (g < S && spl[g] <= k) {
+        lo = g;
+        hi = g + 1;
+        for (int step = 1; hi < S && spl[hi] <= k; hi = lo + step) {
+            lo = hi;
+            step <<= 1;
+        }
+        hi = min(hi, S);
+    } else {
+        hi = g;
+        lo = g - 1;
+        for (int step = 1; lo >= 0 && spl[lo] > k; lo = hi - step) {
+            hi = lo;
+            step <<= 1;
+        }
+        lo = max(lo, -1);
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (spl[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+// Bucket windows: a block's bodies (consecutive in the last tree's order)
+// expect buckets near first / kBucketT + 1; buckets w0 .. w0 + kWin - 1 (w0
+// = that - kWin / 4) are searched in an LDS copy of their splitters and
+// counted in LDS bins; a body outside the window is searched in global
+// memory and goes to its bucket's counter directly.
+constexpr int kWinPer = 8;
+constexpr int kWinBodies = 256 * kWinPer;
+constexpr int kWin = 64;
+
+// ws[0 .. kWin]: spl[w0 - 1 .. w0 + kWin - 1], 0 below the first splitter
+// and ~0 past the last (keys use 63 bits); block-wide, ends with a barrier
+__device__ __forceinline__ void load_window(const unsigned long long *__restrict__ spl, int S, int w0,
+                                            unsigned long long *ws)
+{
+    const int t = threadIdx.x;
+    if (t <= kWin) {
+        const int j = w0 - 1 + t;
+        ws[t] = j < 0 ? 0ull : (j >= S ? ~0ull : spl[j]);
+    }
+    __syncthreads();
+}
+
+// bucket of k if it is one of w0 .. w0 + kWin - 1, else -1
+__device__ __forceinline__ int bucket_in_window(const unsigned long long *ws, int w0, unsigned long long k)
+{
+    if (k < ws[0] || k >= ws[kWin]) return -1;
+    int lo = 0, hi = kWin; // ws[lo] <= k < ws[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ws[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return w0 - 1 + hi;
+}
+
+// Warm keys: body order[i] (the last tree's order) at place i: its key and
+// value, its bucket (bid) and the bucket counts. A block's 256 bodies expect
+// buckets near i / kBucketT + 1; the window starts kWin / 4 below the first.
+__global__ __launch_bounds__(256) void k_keys_warm(Dev d, const int *__restrict__ order,
+                                                   const unsigned long long *__restrict__ spl, int S,
+                                                   unsigned long long *__restrict__ key, int *__restrict__ val,
+                                                   int *__restrict__ bid, unsigned *__restrict__ bcnt)
+{
+    __shared__ unsigned h[kWin];
+    __shared__ unsigned long long ws[kWin + 1];
+    const int t = threadIdx.x, first = blockIdx.x * 256, i = first + t;
+    const int n = d.n;
+    const Ctl *ctl = d.ctl;
+    if (ctl->error) return;
+    const int w0 = first / kBucketT + 1 - kWin / 4;
+    if (t < kWin) h[t] = 0;
+    unsigned long long k = 0;
+    int b = 0;
+    if (i < n) {
+        b = order[i];
+        const float4 p = d.pm4[b];
+        float cx = ctl->root[0], cy = ctl->root[1], cz = ctl->root[2], r = ctl->radius;
+#pragma unroll
+        for (int L = 0; L < kKeyLevels; ++L) {
+            const int o = key_octant(cx, cy, cz, p);
+            k = (k << 3) | (unsigned long long)o;
+            key_descend(cx, cy, cz, r, o);
+        }
+        key[i] = k;
+        val[i] = b;
+    }
+    load_window(spl, S, w0, ws);
+    if (i < n) {
+        int bk = bucket_in_window(ws, w0, k);
+        if (bk >= 0) {
+            atomicAdd(&h[bk - w0], 1u);
+        } else {
+            bk = bucket_of(spl, S, k, i / kBucketT + 1);
+            atomicAdd(&bcnt[bk], 1u);
+        }
+        bid[i] = bk;
+    }
+    __syncthreads();
+    if (t < kWin && h[t]) atomicAdd(&bcnt[w0 + t], h[t]);
+}
+
+// exclusive scan of the S + 1 bucket counts: bstart[0 .. S+1] (one block)
+__global__ __launch_bounds__(1024) void k_bucket_scan(const Ctl *ctl, const unsigned *__restrict__ bcnt, int S,
+                                                      unsigned *__restrict__ bstart)
+{
+    __shared__ unsigned s[1024];
+    const int t = threadIdx.x;
+    if (ctl->error) return;
+    const int nb = S + 1, per = (nb + 1023) / 1024;
+    unsigned sum = 0;
+    for (int k = 0; k < per; ++k) {
+        const int j = t * per + k;
+        if (j < nb) sum += bcnt[j];
+    }
+    s[t] = sum;
+    __syncthreads();
+    for (int st = 1; st < 1024; st <<= 1) {
+        const unsigned u = t >= st ? s[t - st] : 0u;
+        __syncthreads();
+        s[t] += u;
+        __syncthreads();
+    }
+    unsigned run = s[t] - sum;
+    for (int k = 0; k < per; ++k) {
+        const int j = t * per + k;
+        if (j < nb) {
+            bstart[j] = run;
+            run += bcnt[j];
+        }
+    }
+    if (t == 1023) bstart[nb] = s[1023];
+}
+
+// Each body takes the next place of its bucket: LDS counters for the buckets
+// of the block's window (one global atomic per bucket and block), a global
+// atomic for a bucket outside it.
+__global__ __launch_bounds__(256) void k_bucket_scatter(const Ctl *ctl, const unsigned long long *__restrict__ kin,
+                                                        const int *__restrict__ vin, const int *__restrict__ bid,
+                                                        unsigned long long *__restrict__ kout, int *__restrict__ vout,
+                                                        int n, const unsigned *__restrict__ bstart,
+                                                        unsigned *__restrict__ bfill)
+{
+    __shared__ unsigned h[kWin], hb[kWin];
+    if (ctl->error) return;
+    const int t = threadIdx.x, first = blockIdx.x * kWinBodies;
+    const int w0 = first / kBucketT + 1 - kWin / 4;
+    if (t < kWin) h[t] = 0;
+    unsigned long long k[kWinPer];
+    int v[kWinPer], b[kWinPer];
+#pragma unroll
+    for (int j = 0; j < kWinPer; ++j) {
+        const int i = min(first + j * 256 + t, n - 1);
+        k[j] = kin[i];
+        v[j] = vin[i];
+        b[j] = bid[i];
+    }
+    __syncthreads();
+    unsigned at[kWinPer];
+#pragma unroll
+    for (int j = 0; j < kWinPer; ++j) {
+        if (first + j * 256 + t >= n) continue;
+        const unsigned o = (unsigned)(b[j] - w0);
+        at[j] = o < (unsigned)kWin ? atomicAdd(&h[o], 1u) : bstart[b[j]] + atomicAdd(&bfill[b[j]], 1u);
+    }
+    __syncthreads();
+    if (t < kWin) hb[t] = h[t] ? bstart[w0 + t] + atomicAdd(&bfill[w0 + t], h[t]) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kWinPer; ++j) {
+        if (first + j * 256 + t >= n) continue;
+        const unsigned o = (unsigned)(b[j] - w0);
+        const unsigned pos = o < (unsigned)kWin ? hb[o] + at[j] : at[j];
+        kout[pos] = k[j];
+        vout[pos] = v[j];
+    }
+}
+
+struct RankLds {
+    unsigned cnt[kBsWaves][kRsBins]; // per wave and digit in a round; then their exclusive prefix over the waves
+    unsigned lstart[kRsBins];        // first place of each digit
+    unsigned run[kRsBins];           // places of each digit taken by earlier rounds
+    unsigned tot[kRsBins];           // digit counts; then each round's per-digit total
+    unsigned wsum[kBsWaves];
+};
+
+// exclusive scan over the first 256 threads of the block (each passes v; the
+// others pass anything and ignore the result); every thread must call it
+__device__ __forceinline__ unsigned scan256_excl(unsigned v, unsigned *wsum)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned add = 0;
+    for (int ww = 0; ww < w && ww < 4; ++ww) add += wsum[ww];
+    __syncthreads();
+    return add + x - v;
+}
+
+// One stable pass over m keys by the 8 bits at `shift`: (sk, sv) -> (dk, dv),
+// all m places of dk/dv written. Rounds of kBsThreads keys in input order: a
+// key's place = its digit's first place + the places earlier rounds took + the
+// keys of its digit in lower waves of this round + its rank in its wave (eight
+// ballots of the digit's bits). The pointers are LDS or global.
+template <class SK, class SV, class DK, class DV>
+__device__ void rank_pass(SK sk, SV sv, DK dk, DV dv, int m, int shift, RankLds &R)
+{
+    const int t = threadIdx.x, w = t >> 6;
+    if (t < kRsBins) R.tot[t] = 0;
+    for (int x = t; x < kBsWaves * kRsBins; x += kBsThreads) (&R.cnt[0][0])[x] = 0;
+    __syncthreads();
+    for (int j = t; j < m; j += kBsThreads) atomicAdd(&R.tot[(unsigned)(sk[j] >> shift) & 255u], 1u);
+    __syncthreads();
+    const unsigned ls = scan256_excl(t < kRsBins ? R.tot[t] : 0u, R.wsum);
+    if (t < kRsBins) {
+        R.lstart[t] = ls;
+        R.run[t] = 0;
+    }
+    __syncthreads();
+    const int lane = t & 63;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int r0 = 0; r0 < m; r0 += kBsThreads) {
+        const int j = r0 + t;
+        const bool valid = j < m;
+        const unsigned long long k = valid ? sk[j] : 0ull;
+        const int v = valid ? sv[j] : 0;
+        const unsigned dg = (unsigned)(k >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bal = __ballot((dg >> b) & 1u);
+            peers &= ((dg >> b) & 1u) ? bal : ~bal;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) R.cnt[w][dg] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if (t < kRsBins) {
+            unsigned acc = 0;
+#pragma unroll
+            for (int ww = 0; ww < kBsWaves; ++ww) {
+                const unsigned c = R.cnt[ww][t];
+                R.cnt[ww][t] = acc;
+                acc += c;
+            }
+            R.tot[t] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const unsigned at = R.lstart[dg] + R.run[dg] + R.cnt[w][dg] + (unsigned)rank;
+            dk[at] = k;
+            dv[at] = v;
+        }
+        __syncthreads();
+        if (t < kRsBins) {
+            R.run[t] += R.tot[t];
+#pragma unroll
+            for (int ww = 0; ww < kBsWaves; ++ww) R.cnt[ww][t] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// the highest key bit that differs inside key[0 .. m-1] (-1: all equal)
+template <class SK>
+__device__ int varying_top_bit(SK key, int m, unsigned long long *red)
+{
+    const int t = threadIdx.x;
+    if (t == 0) *red = 0ull;
+    __syncthreads();
+    const unsigned long long k0 = key[0];
+    unsigned long long x = 0;
+    for (int j = t; j < m; j += kBsThreads) x |= key[j] ^ k0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    if ((t & 63) == 0 && x) atomicOr(red, x);
+    __syncthreads();
+    const unsigned long long all = *red;
+    __syncthreads();
+    return all ? 63 - __clzll((long long)all) : -1;
+}
+
+// Bucket blockIdx.x of the S + 1 (blocks of one wave), in place in (key,
+// val): lane l holds elements r * 64 + l (r < kSortPer) in registers. A pass
+// by the 8 bits at `shift`: digit counts (LDS atomics) and their exclusive
+// scan give each digit's first place; then row by row, in element order, a
+// key's place is its digit's next free place plus its rank among the lanes
+// of the row with its digit (eight ballots); the row's lanes move their
+// digit's next place on. The keys go to those places in LDS and come back to
+// the registers in element order. A bucket above kBucketCap is listed in
+// big[1 ..] (big[0] counts them) for k_bucket_sort_big.
+constexpr int kSortPer = kBucketCap / 64;
+
+__global__ __launch_bounds__(64) void k_bucket_sort(const Ctl *ctl, unsigned long long *__restrict__ key,
+                                                    int *__restrict__ val, const unsigned *__restrict__ bstart,
+                                                    int *__restrict__ big)
+{
+    __shared__ unsigned long long sk[kBucketCap];
+    __shared__ int sv[kBucketCap];
+    __shared__ unsigned next[kRsBins];
+    if (ctl->error) return;
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int base = (int)bstart[b], m = (int)bstart[b + 1] - base;
+    if (m <= 1) return;
+    if (m > kBucketCap) {
+        if (lane == 0) big[1 + atomicAdd(&big[0], 1)] = b;
+        return;
+    }
+    unsigned long long k[kSortPer];
+    int v[kSortPer];
+    unsigned long long diff = 0;
+    const unsigned long long k0 = key[base];
+#pragma unroll
+    for (int r = 0; r < kSortPer; ++r) {
+        const int idx = r * 64 + lane;
+        k[r] = idx < m ? key[base + idx] : k0;
+        v[r] = idx < m ? val[base + idx] : 0;
+        diff |= k[r] ^ k0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) diff |= __shfl_xor(diff, o);
+    const int top = diff ? 63 - __clzll((long long)diff) : -1;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int shift = 0; shift <= top; shift += 8) {
+#pragma unroll
+        for (int q = 0; q < kRsBins / 64; ++q) next[q * 64 + lane] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortPer; ++r) {
+            if (r * 64 >= m) break;
+            if (r * 64 + lane < m) atomicAdd(&next[(unsigned)(k[r] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        {
+            // lane l scans digits 4l .. 4l+3
+            unsigned c[4], sum = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c[q] = next[lane * 4 + q];
+                sum += c[q];
+            }
+            unsigned x = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned y = __shfl_up(x, o);
+                if (lane >= o) x += y;
+            }
+            unsigned run = x - sum;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                next[lane * 4 + q] = run;
+                run += c[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortPer; ++r) {
+            if (r * 64 >= m) break; // rows past the bucket: no key
+            const bool valid = r * 64 + lane < m;
+            const unsigned dg = (unsigned)(k[r] >> shift) & 255u;
+            unsigned long long peers = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const unsigned long long bal = __ballot((dg >> bit) & 1u);
+                peers &= ((dg >> bit) & 1u) ? bal : ~bal;
+            }
+            const unsigned rank = (unsigned)__popcll(peers & below);
+            const unsigned at = valid ? next[dg] + rank : 0u;
+            __syncthreads();
+            if (valid) {
+                if (rank == 0) next[dg] += (unsigned)__popcll(peers);
+                sk[at] = k[r];
+                sv[at] = v[r];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kSortPer; ++r) {
+            const int idx = r * 64 + lane;
+            if (idx < m) {
+                k[r] = sk[idx];
+                v[r] = sv[idx];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < kSortPer; ++r) {
+        const int idx = r * 64 + lane;
+        if (idx < m) {
+            key[base + idx] = k[r];
+            val[base + idx] = v[r];
+        }
+    }
+}
+
+// The listed buckets (more than kBucketCap bodies), one block each, in place
+// in (key, val) through (tkey, tval), free n-element buffers.
+constexpr int kBigBlocks = 64;
+
+__global__ __launch_bounds__(kBsThreads) void k_bucket_sort_big(Ctl *ctl, unsigned long long *__restrict__ key,
+                                                                int *__restrict__ val, unsigned long long *__restrict__ tkey,
+                                                                int *__restrict__ tval, const unsigned *__restrict__ bstart,
+                                                                const int *__restrict__ big)
+{
+    __shared__ RankLds R;
+    __shared__ unsigned long long red;
+    if (ctl->error) return;
+    const int t = threadIdx.x, nbig = big[0];
+    for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
+        const int b = big[1 + q];
+        const int base = (int)bstart[b], m = (int)bstart[b + 1] - base;
+        if (t == 0) atomicAdd(&ctl->large_buckets, 1);
+        unsigned long long *k0 = key + base, *k1 = tkey + base;
+        int *v0 = val + base, *v1 = tval + base;
+        const int top = varying_top_bit(k0, m, &red);
+        bool in_tmp = false;
+        for (int shift = 0; shift <= top; shift += 8) {
+            if (in_tmp) rank_pass(k1, v1, k0, v0, m, shift, R);
+            else rank_pass(k0, v0, k1, v1, m, shift, R);
+            in_tmp = !in_tmp;
+        }
+        if (in_tmp) {
+            for (int j = t; j < m; j += kBsThreads) {
+                k0[j] = k1[j];
+                v0[j] = v1[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Runs of equal 21-level keys: their bodies' paths extended to 84 levels
 // (deep[0..2][pos], levels 21 .. 83), each run ordered by them, one thread
 // per run.
@@ -393,13 +889,17 @@ __device__ __forceinline__ int key_digit(const unsigned long long *__restrict__ 
 
 // ---------------------------------------------------------------------------
 // Binary radix tree over the sorted keys (Karras 2012, fig. 4), internal node
-// i of n - 1 (node 0 is the root): its range, the common prefix of its keys
-// in bits, and the parent links of its two children (internal or body).
+// i of n - 1 (node 0 is the root): its range [lo, hi] and split, the parent
+// links of its two children (internal or body), and the octree levels it
+// holds: ktop .. ktop + own - 1, the levels k with 3k in (delta(parent),
+// delta(i)]. delta(parent) is the larger of the prefixes the range shares with
+// its two outside neighbours (the parent extends the range towards the
+// neighbour that shares more; -1 for the root).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bintree(Dev d, const unsigned long long *__restrict__ key,
                                                  const unsigned long long *__restrict__ deep, int *__restrict__ iparent,
-                                                 int *__restrict__ lparent, int *__restrict__ dnode,
-                                                 int *__restrict__ lo_out)
+                                                 int *__restrict__ lparent, int *__restrict__ lo_out,
+                                                 int *__restrict__ own, int *__restrict__ ktop)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.n;
@@ -424,21 +924,11 @@ __global__ __launch_bounds__(256) void k_bintree(Dev d, const unsigned long long
     else iparent[g] = i;
     if (hi == g + 1) lparent[g + 1] = i;
     else iparent[g + 1] = i;
-    dnode[i] = dn;
     lo_out[i] = lo;
     if (i == 0) iparent[0] = -1;
-}
-
-// Octree levels held by internal node i: ktop .. ktop + own - 1, the levels k
-// with 3k in (delta(parent), delta(i)] (from level 0 at the root).
-__global__ __launch_bounds__(256) void k_own(Dev d, const int *__restrict__ iparent, const int *__restrict__ dnode,
-                                             int *__restrict__ own, int *__restrict__ ktop)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = d.n;
-    if (i >= n - 1 || d.ctl->error) return; // no binary tree was built
-    const int kt = i == 0 ? 0 : dnode[iparent[i]] / 3 + 1;
-    int kb = dnode[i] / 3;
+    const int dp = max(key_delta(key, deep, n, lo, lo - 1), key_delta(key, deep, n, hi, hi + 1));
+    const int kt = dp < 0 ? 0 : dp / 3 + 1;
+    int kb = dn / 3;
     if (kb >= kBuildLevelLimit) {
         flag_error(d.ctl, kErrDepth);
         kb = kBuildLevelLimit - 1;
@@ -448,17 +938,87 @@ __global__ __launch_bounds__(256) void k_own(Dev d, const int *__restrict__ ipar
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of n ints (the cells each internal node holds -> the number
-// of its first cell): tiles of 1024 scanned in place with their sums kept,
-// the sums scanned by one block, then added back.
+// Exclusive scan of the cells each internal node holds (-> its first cell):
+// tiles of kScanTile scanned with their sums kept (k_scan_tiles), the sums
+// scanned by one block (k_scan_sums); readers add the two (first_cell).
+// The same passes find the tree's depth: the deepest cell's level + 1 (its
+// children are bodies), per tile and then over the tiles, into the running
+// maximum (buildtree.cl:199) with one atomic.
 // ---------------------------------------------------------------------------
-constexpr int kScanTile = 1024;
+constexpr int kScanPer = 4; // one int4 per thread: coalesced
+constexpr int kScanTile = 256 * kScanPer;
 static_assert(kScanTile == kKeyScanTile, "the engine sizes the tile sums with kKeyScanTile");
 
-__device__ __forceinline__ int block_excl_scan1024(int v, int *s, int *total)
+__device__ __forceinline__ int4 load4(const int *__restrict__ a, int i0, int m)
 {
+    if (i0 + 3 < m) return *reinterpret_cast<const int4 *>(a + i0);
+    return make_int4(i0 < m ? a[i0] : 0, i0 + 1 < m ? a[i0 + 1] : 0, i0 + 2 < m ? a[i0 + 2] : 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const Ctl *ctl, const int *__restrict__ in,
+                                                    const int *__restrict__ ktop, int *__restrict__ out, int m,
+                                                    int *__restrict__ sums, int *__restrict__ tdepth)
+{
+    __shared__ int wsum[4], wdep[4];
+    if (ctl->error) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x * kScanTile + t * kScanPer;
+    const int4 v = load4(in, i0, m), kt = load4(ktop, i0, m);
+    const int acc = v.x + v.y + v.z + v.w;
+    int dep = 0;
+    if (v.x > 0) dep = max(dep, kt.x + v.x);
+    if (v.y > 0) dep = max(dep, kt.y + v.y);
+    if (v.z > 0) dep = max(dep, kt.z + v.z);
+    if (v.w > 0) dep = max(dep, kt.w + v.w);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dep = max(dep, __shfl_xor(dep, o));
+    if (lane == 0) wdep[w] = dep;
+    int x = acc; // inclusive scan of the per-thread totals over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int run = x - acc;
+    for (int ww = 0; ww < w; ++ww) run += wsum[ww];
+    const int4 e = make_int4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+    if (i0 + 3 < m) {
+        *reinterpret_cast<int4 *>(out + i0) = e;
+    } else {
+        if (i0 < m) out[i0] = e.x;
+        if (i0 + 1 < m) out[i0 + 1] = e.y;
+        if (i0 + 2 < m) out[i0 + 2] = e.z;
+    }
+    if (t == 0) {
+        sums[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        tdepth[blockIdx.x] = max(max(wdep[0], wdep[1]), max(wdep[2], wdep[3]));
+    }
+}
+
+// one block: exclusive scan of the tile sums in place (each thread a run of
+// them); the tiles' depths into the running maximum
+__global__ __launch_bounds__(1024) void k_scan_sums(Ctl *ctl, int *__restrict__ sums, int ntiles,
+                                                   const int *__restrict__ tdepth)
+{
+    __shared__ int s[1024];
+    __shared__ int wdep[16];
+    if (ctl->error) return;
     const int t = threadIdx.x;
-    s[t] = v;
+    const int per = (ntiles + 1023) / 1024;
+    int acc = 0, dep = 0;
+    for (int k = 0; k < per; ++k) {
+        const int j = t * per + k;
+        if (j < ntiles) {
+            acc += sums[j];
+            dep = max(dep, tdepth[j]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dep = max(dep, __shfl_xor(dep, o));
+    if ((t & 63) == 0) wdep[t >> 6] = dep;
+    s[t] = acc;
     __syncthreads();
     for (int st = 1; st < 1024; st <<= 1) {
         const int u = t >= st ? s[t - st] : 0;
@@ -466,36 +1026,7 @@ __device__ __forceinline__ int block_excl_scan1024(int v, int *s, int *total)
         s[t] += u;
         __syncthreads();
     }
-    const int r = s[t] - v;
-    if (total) *total = s[1023];
-    __syncthreads();
-    return r;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_tiles(const int *__restrict__ in, int *__restrict__ out, int n,
-                                                     int *__restrict__ sums)
-{
-    __shared__ int s[1024];
-    const int i = blockIdx.x * kScanTile + threadIdx.x;
-    const int v = i < n ? in[i] : 0;
-    int tot;
-    const int e = block_excl_scan1024(v, s, &tot);
-    if (i < n) out[i] = e;
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-
-// one block: exclusive scan of the tile sums in place (each thread a run of them)
-__global__ __launch_bounds__(1024) void k_scan_sums(int *__restrict__ sums, int ntiles)
-{
-    __shared__ int s[1024];
-    const int t = threadIdx.x;
-    const int per = (ntiles + 1023) / 1024;
-    int acc = 0;
-    for (int k = 0; k < per; ++k) {
-        const int j = t * per + k;
-        if (j < ntiles) acc += sums[j];
-    }
-    int run = block_excl_scan1024(acc, s, nullptr);
+    int run = s[t] - acc;
     for (int k = 0; k < per; ++k) {
         const int j = t * per + k;
         if (j < ntiles) {
@@ -504,35 +1035,36 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int *__restrict__ sums, int 
             run += c;
         }
     }
-}
-
-__global__ __launch_bounds__(1024) void k_scan_add(int *__restrict__ out, int n, const int *__restrict__ sums)
-{
-    const int i = blockIdx.x * kScanTile + threadIdx.x;
-    if (i < n) out[i] += sums[blockIdx.x];
+    if (t == 0) {
+        for (int k = 0; k < 16; ++k) dep = max(dep, wdep[k]);
+        if (dep > 0) __hip_atomic_fetch_max(&ctl->max_depth, dep, __ATOMIC_RELAXED, AGENT);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Cells and slots. Cell obase[i] + (k - ktop[i]) is internal node i's cell of
-// level k; the root (level 0, held by binary node 0) is cell 0.
+// Cells and slots. Cell first_cell(i) + (k - ktop[i]) is internal node i's cell
+// of level k; the root (level 0, held by binary node 0) is cell 0.
 // ---------------------------------------------------------------------------
-// The nearest binary ancestor (p itself included) that holds a cell. A node
-// that holds none shares its digit level with its parent, so the walk is a
-// few steps; it is bounded anyway, and -1 (an inconsistent tree) is flagged
-// by the caller.
-__device__ __forceinline__ int owning_ancestor(const int *__restrict__ iparent, const int *__restrict__ own, int p)
+struct KeyTree {
+    const unsigned long long *key, *deep;
+    const int *val, *iparent, *lparent, *lo, *own, *ktop, *obase, *tsum;
+};
+
+__device__ __forceinline__ int first_cell(const KeyTree &kt, int i) { return kt.obase[i] + kt.tsum[i / kScanTile]; }
+
+// The deepest cell of the nearest binary ancestor (p itself included) that
+// holds a cell, or -1 (an inconsistent tree; the caller flags it). A node that
+// holds none shares its level with its parent, so the walk is at most a few
+// steps; it is bounded anyway.
+__device__ __forceinline__ int holding_cell(const KeyTree &kt, int p)
 {
     for (int steps = 0; steps < 4 * kBuildLevelLimit * 3 && p >= 0; ++steps) {
-        if (own[p] != 0) return p;
-        p = iparent[p];
+        const int o = kt.own[p];
+        if (o != 0) return first_cell(kt, p) + o - 1;
+        p = kt.iparent[p];
     }
     return -1;
 }
-
-struct KeyTree {
-    const unsigned long long *key, *deep;
-    const int *val, *iparent, *lparent, *dnode, *lo, *own, *ktop, *obase;
-};
 
 // the cell count, the pool check, and the root's record
 __global__ void k_cell_count(Dev d, KeyTree kt)
@@ -540,7 +1072,7 @@ __global__ void k_cell_count(Dev d, KeyTree kt)
     Ctl *ctl = d.ctl;
     if (ctl->error) return;
     const int n = d.n;
-    const int C = n >= 2 ? kt.obase[n - 2] + kt.own[n - 2] : 1;
+    const int C = n >= 2 ? first_cell(kt, n - 2) + kt.own[n - 2] : 1;
     if (C > d.cap) {
         flag_error(ctl, kErrPool);
         return;
@@ -549,76 +1081,46 @@ __global__ void k_cell_count(Dev d, KeyTree kt)
     d.cinfo[0] = make_int4(-1, 0, (int)ctl->epoch, 0);
 }
 
-__global__ __launch_bounds__(256) void k_cells(Dev d, KeyTree kt)
+// Position g: the cells of internal node g (g < n - 1) -- each in the slot of
+// its octant in its parent, and its record -- and sorted body g, in the
+// deepest cell of its nearest holding ancestor, at the level one below the
+// larger prefix it shares with a neighbour.
+__global__ __launch_bounds__(256) void k_link(Dev d, KeyTree kt)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = d.n;
-    const Ctl *ctl = d.ctl;
-    if (i >= n - 1 || ctl->error) return;
-    const int own = kt.own[i];
-    if (own == 0) return;
-    const uint32_t ep = ctl->epoch;
-    const int top = kt.ktop[i], base = kt.obase[i];
-    for (int k = top; k < top + own; ++k) {
-        const int cid = base + (k - top);
-        if (k == 0) continue; // the root has no parent
-        int par;
-        if (k > top) {
-            par = cid - 1;
-        } else {
-            const int y = owning_ancestor(kt.iparent, kt.own, kt.iparent[i]);
-            if (y < 0) {
-                flag_error(d.ctl, kErrSync);
-                return;
-            }
-            par = kt.obase[y] + kt.own[y] - 1;
-        }
-        const int dg = key_digit(kt.key, kt.deep, n, kt.lo[i], k - 1);
-        d.slot[(size_t)par * 8 + dg] = slot_tag(ep, n + cid);
-        d.cinfo[cid] = make_int4(par, k | (dg << 8), (int)ep, 0);
-    }
-}
-
-// Sorted body g into the deepest cell of its nearest holding ancestor; the
-// running maximum depth (buildtree.cl:199), one atomic per block. The grid
-// strides over the bodies: one atomic per wave on that single word cost 0.75
-// ms at 4M bodies (65,536 waves).
-constexpr int kLeafBlocks = 1024;
-
-__global__ __launch_bounds__(256) void k_leaves(Dev d, KeyTree kt)
-{
-    __shared__ int part[4];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.n;
     Ctl *ctl = d.ctl;
-    int depth = 0;
-    if (!ctl->error) {
-        const uint32_t ep = ctl->epoch;
-        for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
-            int par = 0, lv = 0, y = 0;
-            if (n >= 2) {
-                y = owning_ancestor(kt.iparent, kt.own, kt.lparent[g]);
-                if (y >= 0) {
-                    par = kt.obase[y] + kt.own[y] - 1;
-                    lv = kt.ktop[y] + kt.own[y] - 1;
-                }
+    if (g >= n || ctl->error) return;
+    const uint32_t ep = ctl->epoch;
+    const int own = g < n - 1 ? kt.own[g] : 0;
+    if (own > 0) {
+        const int top = kt.ktop[g], base = first_cell(kt, g), lo = kt.lo[g];
+        int par = top > 0 ? holding_cell(kt, kt.iparent[g]) : -1;
+        if (top > 0 && par < 0) {
+            flag_error(ctl, kErrSync);
+            return;
+        }
+        for (int k = top; k < top + own; ++k) {
+            const int cid = base + (k - top);
+            if (k > 0) {
+                const int dg = key_digit(kt.key, kt.deep, n, lo, k - 1);
+                d.slot[(size_t)par * 8 + dg] = slot_tag(ep, n + cid);
+                d.cinfo[cid] = make_int4(par, k | (dg << 8), (int)ep, 0);
             }
-            if (y < 0) {
-                flag_error(ctl, kErrSync);
-                continue;
-            }
-            const int dg = key_digit(kt.key, kt.deep, n, g, lv);
-            d.slot[(size_t)par * 8 + dg] = slot_tag(ep, kt.val[g]);
-            depth = max(depth, lv + 1);
+            par = cid;
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) depth = max(depth, __shfl_xor(depth, o));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = depth;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        depth = max(max(part[0], part[1]), max(part[2], part[3]));
-        if (depth > 0) __hip_atomic_fetch_max(&ctl->max_depth, depth, __ATOMIC_RELAXED, AGENT);
+    int par = 0, lv = 0;
+    if (n >= 2) {
+        par = holding_cell(kt, kt.lparent[g]);
+        if (par < 0) {
+            flag_error(ctl, kErrSync);
+            return;
+        }
+        lv = max(key_delta(kt.key, kt.deep, n, g, g - 1), key_delta(kt.key, kt.deep, n, g, g + 1)) / 3;
     }
+    const int dg = key_digit(kt.key, kt.deep, n, g, lv);
+    d.slot[(size_t)par * 8 + dg] = slot_tag(ep, kt.val[g]);
 }
 
 // Stage 4 (sort.cl:12-76) on the key-sorted tree: the octant-path order is the
@@ -636,43 +1138,60 @@ __global__ __launch_bounds__(256) void k_sort_keys(Dev d, const int *__restrict_
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-hipError_t launch_build_keys(const Dev &d, hipStream_t st)
+hipError_t launch_build_keys(const Dev &d, bool warm, hipStream_t st)
 {
     const int n = d.n;
     const dim3 b256(256), g256((n + 255) / 256);
-    const int ntiles = (n + kRsTile - 1) / kRsTile;
     unsigned long long *ka = d.kb_key[0], *kb = d.kb_key[1];
     int *va = d.kb_val[0], *vb = d.kb_val[1];
-    hipLaunchKernelGGL(k_keys, g256, b256, 0, st, d, ka, va);
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_rs_count, dim3(ntiles), dim3(kRsThreads), 0, st, ka, n, shift, d.kb_hist, ntiles);
-        hipLaunchKernelGGL(k_rs_offsets, dim3(kRsBins), dim3(1024), 0, st, d.kb_hist, ntiles, d.kb_offs, d.kb_tot);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, st, ka, va, kb, vb, n, shift, d.kb_hist,
-                           d.kb_offs, d.kb_tot, ntiles);
-        unsigned long long *tk = ka;
-        ka = kb;
-        kb = tk;
-        int *tv = va;
-        va = vb;
-        vb = tv;
+    if (warm) {
+        // keys in the last tree's order into [1], splitters from the last
+        // tree's sorted keys in [0], bucketed into [0], each bucket sorted in
+        // place ([1] is the spill space of large buckets)
+        // (the buckets of the bodies go to kb_lo, which the binary tree fills later)
+        const int S = bucket_samples(n);
+        const dim3 gw((n + kWinBodies - 1) / kWinBodies);
+        hipLaunchKernelGGL(k_splitters, dim3((S + 1 + 255) / 256), b256, 0, st, d.ctl, ka, S, d.kb_spl, d.kb_bcnt,
+                           d.kb_bfill, d.kb_big);
+        hipLaunchKernelGGL(k_keys_warm, g256, b256, 0, st, d, d.sorted, d.kb_spl, S, kb, vb, d.kb_lo, d.kb_bcnt);
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, d.ctl, d.kb_bcnt, S, d.kb_bstart);
+        hipLaunchKernelGGL(k_bucket_scatter, gw, b256, 0, st, d.ctl, kb, vb, d.kb_lo, ka, va, n, d.kb_bstart,
+                           d.kb_bfill);
+        hipLaunchKernelGGL(k_bucket_sort, dim3(S + 1), dim3(64), 0, st, d.ctl, ka, va, d.kb_bstart, d.kb_big);
+        hipLaunchKernelGGL(k_bucket_sort_big, dim3(kBigBlocks), dim3(kBsThreads), 0, st, d.ctl, ka, va, kb, vb,
+                           d.kb_bstart, d.kb_big);
+    } else {
+        const int ntiles = (n + kRsTile - 1) / kRsTile;
+        hipLaunchKernelGGL(k_keys, g256, b256, 0, st, d, nullptr, ka, va);
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 8 * pass;
+            hipLaunchKernelGGL(k_rs_count, dim3(ntiles), dim3(kRsThreads), 0, st, ka, n, shift, d.kb_hist, ntiles);
+            hipLaunchKernelGGL(k_rs_offsets, dim3(kRsBins), dim3(1024), 0, st, d.kb_hist, ntiles, d.kb_offs, d.kb_tot);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, st, ka, va, kb, vb, n, shift, d.kb_hist,
+                               d.kb_offs, d.kb_tot, ntiles);
+            unsigned long long *tk = ka;
+            ka = kb;
+            kb = tk;
+            int *tv = va;
+            va = vb;
+            vb = tv;
+        }
+        // eight passes: the sorted keys are back in kb_key[0] / kb_val[0]
     }
-    // eight passes: the sorted keys are back in kb_key[0] / kb_val[0]; the
-    // other pair and the binary tree's arrays (written later) serve long runs
+    // the other pair and the binary tree's arrays (written later) serve long runs
     hipLaunchKernelGGL(k_deep_runs, g256, b256, 0, st, d, ka, va, d.kb_deep, d.kb_ipar, d.kb_lpar, kb, vb);
-    KeyTree kt{ka, d.kb_deep, va, d.kb_ipar, d.kb_lpar, d.kb_dnode, d.kb_lo, d.kb_own, d.kb_ktop, d.kb_obase};
+    KeyTree kt{ka, d.kb_deep, va, d.kb_ipar, d.kb_lpar, d.kb_lo, d.kb_own, d.kb_ktop, d.kb_obase, d.kb_scan};
     if (n >= 2) {
         const dim3 gi((n - 1 + 255) / 256);
-        hipLaunchKernelGGL(k_bintree, gi, b256, 0, st, d, ka, d.kb_deep, d.kb_ipar, d.kb_lpar, d.kb_dnode, d.kb_lo);
-        hipLaunchKernelGGL(k_own, gi, b256, 0, st, d, d.kb_ipar, d.kb_dnode, d.kb_own, d.kb_ktop);
+        hipLaunchKernelGGL(k_bintree, gi, b256, 0, st, d, ka, d.kb_deep, d.kb_ipar, d.kb_lpar, d.kb_lo, d.kb_own,
+                           d.kb_ktop);
         const int m = n - 1, nt = (m + kScanTile - 1) / kScanTile;
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanTile), 0, st, d.kb_own, d.kb_obase, m, d.kb_scan);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, d.kb_scan, nt);
-        hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(kScanTile), 0, st, d.kb_obase, m, d.kb_scan);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, st, d.ctl, d.kb_own, d.kb_ktop, d.kb_obase, m,
+                           d.kb_scan, d.kb_tdep);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, d.ctl, d.kb_scan, nt, d.kb_tdep);
     }
     hipLaunchKernelGGL(k_cell_count, dim3(1), dim3(1), 0, st, d, kt);
-    if (n >= 2) hipLaunchKernelGGL(k_cells, dim3((n - 1 + 255) / 256), b256, 0, st, d, kt);
-    hipLaunchKernelGGL(k_leaves, dim3(min((n + 255) / 256, kLeafBlocks)), b256, 0, st, d, kt);
+    hipLaunchKernelGGL(k_link, g256, b256, 0, st, d, kt);
     return hipGetLastError();
 }
 

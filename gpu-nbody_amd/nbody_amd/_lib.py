@@ -74,7 +74,7 @@ class NbCounters(C.Structure):
         ("check_ms", C.c_double),
         ("checks", C.c_int64),
         ("coarse_alpha", C.c_double),
-        ("reserved", C.c_int64 * 1),
+        ("large_buckets", C.c_int64),
     ]
 
 

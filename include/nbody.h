@@ -170,7 +170,7 @@ int nb_get_cells(nb_sim *sim, int32_t cap, int32_t *ncells, int32_t *count, floa
 
 typedef struct nb_counters {
     int64_t step;          /* the reference's _step counter */
-    int64_t cells;         /* cells allocated by the last build (incl. unused spares) */
+    int64_t cells;         /* cells of the last build */
     int64_t max_depth;     /* running maximum depth (reference _maxDepth) */
     int64_t interactions;  /* body-node interactions of the last force stage */
     int64_t error;         /* device error word */
@@ -184,7 +184,9 @@ typedef struct nb_counters {
                               and gather) since nb_create, cumulative */
     int64_t checks;        /* learned mode: sampled checks run since nb_create */
     double coarse_alpha;   /* learned mode: the relative test's alpha in effect (check_adapt) */
-    int64_t reserved[1];
+    int64_t large_buckets; /* builds from the last tree's order: buckets too large to sort on
+                              chip (sorted through device memory), cumulative since the last
+                              nb_set_state */
 } nb_counters;
 int nb_get_counters(nb_sim *sim, nb_counters *out);
 

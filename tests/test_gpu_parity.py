@@ -111,6 +111,57 @@ def test_summarize_and_sort_bitexact(nb, oracle, case):
     sim.close()
 
 
+def _collapse(n=32768, seed=12):
+    """n - 8 bodies uniform in [-1, 1]^3 heading for a ball of radius 0.01
+    around (0.3, 0.3, 0.3), which they reach after a step of dt 1 (masses 1e-9:
+    gravity moves them by less than 1e-9), and 8 bodies at rest on the corners,
+    which keep the bounding box."""
+    g = np.random.default_rng(seed)
+    p = g.uniform(-1.0, 1.0, (n, 3))
+    p[:8] = [[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+    target = 0.3 + g.uniform(-0.01, 0.01, (n, 3))
+    v = target - p
+    v[:8] = 0.0
+    m = np.full(n, 1e-9)
+    return tuple(np.ascontiguousarray(a, np.float32) for a in (p[:, 0], p[:, 1], p[:, 2], v[:, 0], v[:, 1], v[:, 2], m))
+
+
+@pytest.mark.parametrize("case,dt,large", [("plummer_32k", 0.0, False), ("plummer_32k", 1e-3, False),
+                                           ("collapse_32k", 1.0, True)])
+def test_warm_build_tracks_oracle(nb, oracle, case, dt, large):
+    """After a step the build starts from that tree's body order (sorted[]):
+    bucketed by splitters taken from that tree's sorted keys, each bucket
+    sorted on chip (or through device memory when it is large). The second
+    tree against the oracle's tree of the same positions: membership, cells
+    and sorted[] bit for bit. dt 0 leaves the positions (and the order)
+    unchanged; in the collapse case nearly every body moves into a ball that
+    one bucket of the last tree covers, which is then far too large for LDS."""
+    st = _collapse() if case == "collapse_32k" else _universes()[case]()
+    sim = make_sim(st, theta=0.5, softening=0.05, dt=dt)
+    sim.steps(1)
+    x, y, z, vx, vy, vz = sim.state()
+    orc = make_oracle((x, y, z, vx, vy, vz, st[6]), 0.5, 0.05, dt, 16)
+    before = sim.counters().large_buckets
+    for stage in (1, 2):
+        sim.run_stage(stage)
+    assert orc.run_stages(1, 2) == 0
+    gp, gd = sim.paths()
+    op, od = orc.paths()
+    assert np.all(gd > 0)
+    assert np.array_equal(gd, od)
+    assert np.array_equal(gp, op)
+    for stage in (3, 4):
+        sim.run_stage(stage)
+    assert orc.run_stages(3, 4) == 0
+    gc, oc = sim.cells(), orc.cells()
+    assert np.array_equal(cell_multiset(gc["count"], gc["x"], gc["y"], gc["z"], gc["m"]),
+                          cell_multiset(oc["count"], oc["x"], oc["y"], oc["z"], oc["m"]))
+    assert np.array_equal(sim.sorted(), orc.sorted())
+    grew = sim.counters().large_buckets - before
+    assert (grew > 0) == large, grew
+    sim.close()
+
+
 @pytest.mark.parametrize("vote", [16, 32, 64])
 @pytest.mark.parametrize("case", CASES)
 def test_force_parity(nb, oracle, case, vote):
