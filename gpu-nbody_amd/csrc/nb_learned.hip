@@ -278,88 +278,108 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             q[k] = cdp[k];
             if (!kGeo) mc[k] = cmac[(size_t)c * 8 + k];
         }
-        // per popped cell: bodies, cell monopoles and cell quadrupoles summed
-        // separately (the features need the parts), then one compensated add
-        float pbx = 0.f, pby = 0.f, pbz = 0.f, pmx = 0.f, pmy = 0.f, pmz = 0.f;
-        float pqx = 0.f, pqy = 0.f, pqz = 0.f;
-        int cnt = 0;
+        // First the opening decisions, wave-uniform bit masks of the children
+        // (children are compacted to the front: past the first negative entry
+        // every one is negative). A cell: the opening test of this traversal.
+        // The coarse one takes the extent test first (threshold b^2 /
+        // theta_c^2 + eps^2 from summarize) and works out the relative test,
+        // with the fifth power of d2, only when every lane passed it: both
+        // votes are wave-uniform, so the short circuit is a scalar branch.
+        // Lane masks come straight from the compares (3 = ordered >=, 5 =
+        // ordered <=). An opened cell's stack entry goes into lane `npush` of
+        // `push` (a lane write, no branch on the lane); all of them are stored
+        // after the loop.
+        unsigned bodies = 0, cells = 0;
         int npush = 0;
         bool badv = false;
         unsigned push = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            // children are compacted to the front: past the first negative
-            // entry every one is negative (skipped, no early exit)
             const int v = ch[k];
             if (v < 0) continue;
+            if (v < n) {
+                bodies |= 1u << k;
+                continue;
+            }
             const float dx = q[k].x - me.x, dy = q[k].y - me.y, dz = q[k].z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-            if (v < n) {
-                const float ri = __builtin_amdgcn_rsqf(d2);
-                const float f = q[k].w * (ri * ri * ri);
-                pbx = fmaf(dx, f, pbx);
-                pby = fmaf(dy, f, pby);
-                pbz = fmaf(dz, f, pbz);
-                ++cnt;
-                if (kFeatOn) {
-                    ++nbod;
-                    smb += q[k].w;
-                }
-            } else {
-                // a cell: the opening test of this traversal. The coarse one takes
-                // the extent test first (threshold b^2 / theta_c^2 + eps^2 from
-                // summarize) and works out the relative test, with the fifth power
-                // of d2, only when every lane passed it: both votes are
-                // wave-uniform, so the short circuit is a scalar branch. Lane
-                // masks come straight from the compares (3 = ordered >=, 5 =
-                // ordered <=).
-                const uint64_t exec = __builtin_amdgcn_read_exec();
-                bool take = __builtin_amdgcn_fcmpf(d2, kGeo ? thr_geo : mc[k].x, 3) == exec;
-                if (!kGeo && take) {
-                    float p = d2 * d2;
-                    p = p * p;
-                    p = p * d2;
-                    take = __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5) == exec;
-                }
-                if (take) {
-                    const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qzz, T eps^2)
-                    const float4 qb = cqb[(size_t)c * 8 + k]; // (Qxy, Qxz, Qyz, 0)
-                    const float ri = __builtin_amdgcn_rsqf(d2);
-                    const float r2i = ri * ri;
-                    const float r3 = r2i * ri;
-                    const float f = q[k].w * r3;
-                    const float Qx = fmaf(qa.x, dx, fmaf(qb.x, dy, qb.y * dz));
-                    const float Qy = fmaf(qb.x, dx, fmaf(qa.y, dy, qb.z * dz));
-                    const float Qz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.z * dz));
-                    const float dQd = fmaf(Qx, dx, fmaf(Qy, dy, Qz * dz));
-                    const float r5 = r3 * r2i;
-                    const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
-                    ++cnt;
-                    ++ncell;
-                    if (kFeatOn) {
-                        pmx = fmaf(dx, f, pmx);
-                        pmy = fmaf(dy, f, pmy);
-                        pmz = fmaf(dz, f, pmz);
-                        pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
-                        pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
-                        pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
-                        ++nc;
-                    } else {
-                        // monopole and quadrupole radial terms share one factor
-                        const float h = f + c2;
-                        pmx = fmaf(dx, h, fmaf(-r5, Qx, pmx));
-                        pmy = fmaf(dy, h, fmaf(-r5, Qy, pmy));
-                        pmz = fmaf(dz, h, fmaf(-r5, Qz, pmz));
-                    }
-                } else {
-                    // opened: its stack entry goes into lane `npush` of `push`
-                    // (a lane write, no branch on the lane), all of them are
-                    // stored after the loop
-                    push = lane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
-                    ++npush;
-                    badv |= v - n >= cap;
-                }
+            const uint64_t exec = __builtin_amdgcn_read_exec();
+            bool take = __builtin_amdgcn_fcmpf(d2, kGeo ? thr_geo : mc[k].x, 3) == exec;
+            if (!kGeo && take) {
+                float p = d2 * d2;
+                p = p * p;
+                p = p * d2;
+                take = __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5) == exec;
             }
+            if (take) {
+                cells |= 1u << k;
+            } else {
+                push = lane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
+                ++npush;
+                badv |= v - n >= cap;
+            }
+        }
+        // Then the interactions, each kind in child order, each branch only
+        // around its own work, so the running sums are updated in place (a
+        // branch that both accumulates and pushes makes the compiler copy
+        // them at every child): bodies, cell monopoles and cell quadrupoles
+        // summed separately (the features need the parts), then one
+        // compensated add. d is worked out again, with the same operations as
+        // for the test.
+        float pbx = 0.f, pby = 0.f, pbz = 0.f, pmx = 0.f, pmy = 0.f, pmz = 0.f;
+        float pqx = 0.f, pqy = 0.f, pqz = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (!((bodies >> k) & 1u)) continue;
+            const float4 qk = q[k];
+            const float dx = qk.x - me.x, dy = qk.y - me.y, dz = qk.z - me.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            const float ri = __builtin_amdgcn_rsqf(d2);
+            const float f = qk.w * (ri * ri * ri);
+            pbx = fmaf(dx, f, pbx);
+            pby = fmaf(dy, f, pby);
+            pbz = fmaf(dz, f, pbz);
+            if (kFeatOn) smb += qk.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (!((cells >> k) & 1u)) continue;
+            const float4 qk = q[k];
+            const float dx = qk.x - me.x, dy = qk.y - me.y, dz = qk.z - me.z;
+            const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qzz, T eps^2)
+            const float4 qb = cqb[(size_t)c * 8 + k]; // (Qxy, Qxz, Qyz, 0)
+            const float ri = __builtin_amdgcn_rsqf(d2);
+            const float r2i = ri * ri;
+            const float r3 = r2i * ri;
+            const float f = qk.w * r3;
+            const float Qx = fmaf(qa.x, dx, fmaf(qb.x, dy, qb.y * dz));
+            const float Qy = fmaf(qb.x, dx, fmaf(qa.y, dy, qb.z * dz));
+            const float Qz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.z * dz));
+            const float dQd = fmaf(Qx, dx, fmaf(Qy, dy, Qz * dz));
+            const float r5 = r3 * r2i;
+            const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
+            if (kFeatOn) {
+                pmx = fmaf(dx, f, pmx);
+                pmy = fmaf(dy, f, pmy);
+                pmz = fmaf(dz, f, pmz);
+                pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
+                pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
+                pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
+            } else {
+                // monopole and quadrupole radial terms share one factor
+                const float h = f + c2;
+                pmx = fmaf(dx, h, fmaf(-r5, Qx, pmx));
+                pmy = fmaf(dy, h, fmaf(-r5, Qy, pmy));
+                pmz = fmaf(dz, h, fmaf(-r5, Qz, pmz));
+            }
+        }
+        const int nb_here = __popc(bodies), nc_here = __popc(cells);
+        const int cnt = nb_here + nc_here;
+        ncell += nc_here;
+        if (kFeatOn) {
+            nbod += nb_here;
+            nc += nc_here;
         }
         // the opened children in one store, in child order (the last one is
         // popped first); the tree is at most kMaxDepth deep and the pool holds
