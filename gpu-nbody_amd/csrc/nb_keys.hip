@@ -1084,7 +1084,16 @@ __global__ void k_cell_count(Dev d, KeyTree kt)
 // Position g: the cells of internal node g (g < n - 1) -- each in the slot of
 // its octant in its parent, and its record -- and sorted body g, in the
 // deepest cell of its nearest holding ancestor, at the level one below the
-// larger prefix it shares with a neighbour.
+// larger prefix it shares with a neighbour. The loads are issued in two
+// rounds of independent reads (clamped indices, no branch in between): the
+// kernel is bound by their latency, not by its arithmetic.
+__device__ __forceinline__ int deepest_of(const KeyTree &kt, int own, int ob, int ts, int p)
+{
+    // the holding cell of binary node p, given p's own, obase and tile sum:
+    // p itself when it holds cells, else the walk up from its parent
+    return own != 0 ? ob + ts + own - 1 : (p >= 0 ? holding_cell(kt, kt.iparent[p]) : -1);
+}
+
 __global__ __launch_bounds__(256) void k_link(Dev d, KeyTree kt)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1092,35 +1101,56 @@ __global__ __launch_bounds__(256) void k_link(Dev d, KeyTree kt)
     Ctl *ctl = d.ctl;
     if (g >= n || ctl->error) return;
     const uint32_t ep = ctl->epoch;
-    const int own = g < n - 1 ? kt.own[g] : 0;
-    if (own > 0) {
-        const int top = kt.ktop[g], base = first_cell(kt, g), lo = kt.lo[g];
-        int par = top > 0 ? holding_cell(kt, kt.iparent[g]) : -1;
-        if (top > 0 && par < 0) {
-            flag_error(ctl, kErrSync);
-            return;
+    if (n < 2) { // the root holds the one body
+        d.slot[key_digit(kt.key, kt.deep, n, 0, 0)] = slot_tag(ep, kt.val[0]);
+        return;
+    }
+    // round 1: this position's node and leaf records and the keys around it
+    const int gi = min(g, n - 2);
+    const int own = kt.own[gi], top = kt.ktop[gi], lo = kt.lo[gi], ip = kt.iparent[gi];
+    const int ob = kt.obase[gi], ts = kt.tsum[gi / kScanTile];
+    const int lp = kt.lparent[g];
+    const int body = kt.val[g];
+    const unsigned long long kg = kt.key[g], km = kt.key[max(g - 1, 0)], kq = kt.key[min(g + 1, n - 1)];
+    // round 2: the parents' records, and the key of the node's first body
+    const int ipc = max(ip, 0);
+    const int own_ip = kt.own[ipc], ob_ip = kt.obase[ipc], ts_ip = kt.tsum[ipc / kScanTile];
+    const int own_lp = kt.own[lp], ob_lp = kt.obase[lp], ts_lp = kt.tsum[lp / kScanTile];
+    const unsigned long long klo = kt.key[lo];
+    if (g < n - 1 && own > 0) {
+        int par = -1;
+        if (top > 0) {
+            par = deepest_of(kt, own_ip, ob_ip, ts_ip, ip);
+            if (par < 0) {
+                flag_error(ctl, kErrSync);
+                return;
+            }
         }
+        const int base = ob + ts;
         for (int k = top; k < top + own; ++k) {
             const int cid = base + (k - top);
             if (k > 0) {
-                const int dg = key_digit(kt.key, kt.deep, n, lo, k - 1);
+                const int L = k - 1;
+                const int dg = L < kKeyLevels ? (int)((klo >> (3 * (kKeyLevels - 1 - L))) & 7ull)
+                                              : key_digit(kt.key, kt.deep, n, lo, L);
                 d.slot[(size_t)par * 8 + dg] = slot_tag(ep, n + cid);
                 d.cinfo[cid] = make_int4(par, k | (dg << 8), (int)ep, 0);
             }
             par = cid;
         }
     }
-    int par = 0, lv = 0;
-    if (n >= 2) {
-        par = holding_cell(kt, kt.lparent[g]);
-        if (par < 0) {
-            flag_error(ctl, kErrSync);
-            return;
-        }
-        lv = max(key_delta(kt.key, kt.deep, n, g, g - 1), key_delta(kt.key, kt.deep, n, g, g + 1)) / 3;
+    const int par = deepest_of(kt, own_lp, ob_lp, ts_lp, lp);
+    if (par < 0) {
+        flag_error(ctl, kErrSync);
+        return;
     }
-    const int dg = key_digit(kt.key, kt.deep, n, g, lv);
-    d.slot[(size_t)par * 8 + dg] = slot_tag(ep, kt.val[g]);
+    // common prefixes with the neighbours from the keys in hand (the deep
+    // words only when a key is repeated)
+    const int dm = g == 0 ? -1 : (kg != km ? __clzll((long long)(kg ^ km)) - 1 : key_delta(kt.key, kt.deep, n, g, g - 1));
+    const int dq = g == n - 1 ? -1 : (kg != kq ? __clzll((long long)(kg ^ kq)) - 1 : key_delta(kt.key, kt.deep, n, g, g + 1));
+    const int lv = max(dm, dq) / 3;
+    const int dg = lv < kKeyLevels ? (int)((kg >> (3 * (kKeyLevels - 1 - lv))) & 7ull) : key_digit(kt.key, kt.deep, n, g, lv);
+    d.slot[(size_t)par * 8 + dg] = slot_tag(ep, body);
 }
 
 // Stage 4 (sort.cl:12-76) on the key-sorted tree: the octant-path order is the
