@@ -56,7 +56,7 @@ __device__ __forceinline__ void finish_body(const Dev &d, int pos, float4 me, fl
     if (mode == 1) {
         const float dvx = (fx * dt) * 0.5f, dvy = (fy * dt) * 0.5f, dvz = (fz * dt) * 0.5f;
         const float vhx = vx + dvx, vhy = vy + dvy, vhz = vz + dvz;
-        float4 p = me; // this body's (x, y, z, m), from the sorted copy
+        float4 p = me; // this body's (x, y, z, m)
         if (d.fp64) {
             const double4 q = d.pos64[b];
             const double x = fma((double)vhx, (double)dt, q.x);
@@ -70,6 +70,7 @@ __device__ __forceinline__ void finish_body(const Dev &d, int pos, float4 me, fl
             p.z = fmaf(vhz, dt, p.z);
         }
         d.pm4[b] = p;
+        d.bnode[pos] = p; // the new record in sorted order, read by the next build
         vx = vhx + dvx;
         vy = vhy + dvy;
         vz = vhz + dvz;

@@ -74,6 +74,7 @@ struct nb_sim {
     hipEvent_t ord_in = nullptr, ord_out = nullptr;
     bool ord_ready = false; // d.gorder holds an order once ord_out has passed
     bool keys_warm = false; // d.sorted holds the last tree's body order (the build starts from it)
+    bool staged = false;    // d.bnode holds pm4 in d.sorted order (the last fused update wrote it)
     // copyVertices staging: 2 N float4 (positions, then velocities), allocated
     // by the first nb_copy_vertices call and kept until nb_destroy
     float4 *vtx = nullptr;
@@ -284,16 +285,20 @@ int launch_force_method(nb_sim *s, int method, int fmode)
 {
     const nb::Dev &d = s->d;
     hipError_t e = hipSuccess;
+    int r = NB_OK;
     switch (method) {
     case NB_MODE_PLAIN: e = nb::launch_force(d, s->p.vote_width, fmode, s->stream); break;
-    case NB_MODE_COARSE: return launch_coarse_ordered(s, nb::kOutFinish, fmode);
+    case NB_MODE_COARSE: r = launch_coarse_ordered(s, nb::kOutFinish, fmode); break;
     case NB_MODE_LEARNED:
         if (!s->have_model) return fail(s, NB_E_MODEL, "learned mode without a model (nb_train or nb_set_model)");
-        return launch_coarse_ordered(s, nb::kOutLearned, fmode);
+        r = launch_coarse_ordered(s, nb::kOutLearned, fmode);
+        break;
     default: return fail(s, NB_E_ARG, "unknown force method %d", method);
     }
     if (e != hipSuccess) return fail(s, NB_E_HIP, "force launch: %s", hipGetErrorString(e));
-    return NB_OK;
+    // the fused update writes every body's new record at its sorted place too
+    if (!r && fmode == 1) s->staged = true;
+    return r;
 }
 
 int launch_stage(nb_sim *s, int stage)
@@ -303,7 +308,7 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->keys_warm, s->stream); break;
+    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->keys_warm, s->keys_warm && s->staged, s->stream); break;
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
@@ -316,10 +321,14 @@ int launch_stage(nb_sim *s, int stage)
     case NB_STAGE_SORT:
         e = nb::launch_sort_keys(d, s->stream);
         s->keys_warm = true; // sorted[] holds this tree's body order for the next build
+        s->staged = false;   // until the fused update writes bnode[] in that order
         break;
     case NB_STAGE_FORCE: return launch_force_method(s, force_method(s), fused);
     case NB_STAGE_INTEGRATE:
-        if (!fused) e = nb::launch_integrate(d, s->stream);
+        if (!fused) {
+            e = nb::launch_integrate(d, s->stream);
+            s->staged = false;
+        }
         break;
     default: return fail(s, NB_E_ARG, "stage %d out of range 1..6", stage);
     }
@@ -849,6 +858,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     s->hint_valid = false; // the first step reads its level sizes back
     s->ord_ready = false;  // the launch order of the coarse traversal belongs to the old bodies
     s->keys_warm = false;  // and so does sorted[]: the next build sorts from scratch
+    s->staged = false;
     reset_control(s);
     s->have_state = true;
     s->err.clear();
@@ -909,9 +919,11 @@ static int step_loop(nb_sim *s, int32_t nsteps)
             int r;
             if (check && st == NB_STAGE_FORCE)
                 r = checked_force(s);
-            else if (check && st == NB_STAGE_INTEGRATE)
+            else if (check && st == NB_STAGE_INTEGRATE) {
                 r = nb::launch_integrate(s->d, s->stream) == hipSuccess ? NB_OK
                                                                          : fail(s, NB_E_HIP, "integrate launch");
+                s->staged = false;
+            }
             else
                 r = launch_stage(s, st);
             if (r) return r;

@@ -27,8 +27,7 @@ namespace nb {
 // Vote width 64: one wave, one group, uniform traversal.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ cchild,
-                                                 const float4 *__restrict__ cdata,
-                                                 const float4 *__restrict__ bnode, int mode)
+                                                 const float4 *__restrict__ cdata, int mode)
 {
     __shared__ unsigned stk[4][kStack];
     __shared__ float dq_s[kMaxDepth];
@@ -50,7 +49,7 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
     const bool valid = pos < n;
     // a lane past the last body sits at infinity: its d^2 is infinite, so it
     // accepts every cell and never holds up the vote (its sums are discarded)
-    const float4 me = valid ? bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
+    const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
 
@@ -160,7 +159,7 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
     const bool valid = pos < n;
     // a lane past the last body sits at infinity: its d^2 is infinite, so it
     // accepts every cell and never holds up the vote (its sums are discarded)
-    const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
+    const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const unsigned long long gmask = ((1ull << G) - 1ull) << ((lane / G) * G);
     const int glane = lane % G; // this lane's place in its group
@@ -240,8 +239,7 @@ hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
     case 16: hipLaunchKernelGGL(k_force_g<16>, grid, block, 0, st, d, mode); break;
     case 32: hipLaunchKernelGGL(k_force_g<32>, grid, block, 0, st, d, mode); break;
     case 64:
-        hipLaunchKernelGGL(k_force64, grid, block, 0, st, d, (const int *)d.cchild, (const float4 *)d.cdata,
-                           (const float4 *)d.bnode, mode);
+        hipLaunchKernelGGL(k_force64, grid, block, 0, st, d, (const int *)d.cchild, (const float4 *)d.cdata, mode);
         break;
     default: return hipErrorInvalidValue;
     }

@@ -25,7 +25,8 @@
 //                                   (float2[8*cap], the opening-test constants)
 //                                   written for cell children only
 //   levels   lvl_list : int32[cap]  live cells grouped by level
-//   sorted   bnode  : float4[N]     (x, y, z, m) of the bodies in sorted order
+//   sorted   bnode  : float4[N]     (x, y, z, m) in sorted order as the fused update
+//                                   wrote them (the next build's positions)
 //            sorted : int32[N]      body index at each sorted position
 //   build    kb_key, kb_val : uint64[N] and int32[N], two of each: octant-path
 //                                   keys and body indices through the radix
@@ -142,8 +143,10 @@ struct Dev {
 // ---- launchers (nb_tree.hip, nb_force.hip) ---------------------------------
 hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
 // the build from sorted octant-path keys (the same child slots and cell records)
-hipError_t launch_build_keys(const Dev &d, bool warm, hipStream_t st);
-// stage 4 after the key-sorted build: sorted[] and bnode[] from its body order
+// warm: start from sorted[] and the last sorted keys; staged: bnode[] holds
+// the records in sorted[] order
+hipError_t launch_build_keys(const Dev &d, bool warm, bool staged, hipStream_t st);
+// stage 4 after the key-sorted build: sorted[] from its body order
 hipError_t launch_sort_keys(const Dev &d, hipStream_t st);
 constexpr int kKeyTile = 2048;     // keys per radix-sort tile (first build)
 constexpr int kKeyScanTile = 1024; // cell counts per scan tile

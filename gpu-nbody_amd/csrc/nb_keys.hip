@@ -356,47 +356,86 @@ __device__ __forceinline__ int bucket_in_window(const unsigned long long *ws, in
     return w0 - 1 + hi;
 }
 
+// The place of this lane among the lanes of its wave that take LDS bin o
+// (those with `take`), after h[o], which moves on by their count: one LDS
+// atomic per bin present in the wave instead of one per lane (the lanes of a
+// wave nearly all share a bin). Every lane of the wave calls it.
+__device__ __forceinline__ unsigned claim_bin(unsigned *h, int o, bool take)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned long long todo = __ballot(take);
+    unsigned mine = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int bb = __shfl(o, leader);
+        const unsigned long long same = __ballot(o == bb) & todo;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&h[bb], (unsigned)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) mine = base + (unsigned)__popcll(same & below);
+        todo &= ~same;
+    }
+    return mine;
+}
+
 // Warm keys: body order[i] (the last tree's order) at place i: its key and
-// value, its bucket (bid) and the bucket counts. A block's 256 bodies expect
-// buckets near i / kBucketT + 1; the window starts kWin / 4 below the first.
+// value, its bucket (bid) and the bucket counts. Its record comes from
+// `staged` (the records in that order, as the last fused update wrote them)
+// when given, else from pm4. A block takes kWarmPer * 256 places, whose
+// buckets are expected near first / kBucketT + 1; the window starts kWin / 4
+// below. Every load is issued before the arithmetic.
+constexpr int kWarmPer = 4;
+constexpr int kWarmBodies = 256 * kWarmPer;
+
 __global__ __launch_bounds__(256) void k_keys_warm(Dev d, const int *__restrict__ order,
+                                                   const float4 *__restrict__ staged,
                                                    const unsigned long long *__restrict__ spl, int S,
                                                    unsigned long long *__restrict__ key, int *__restrict__ val,
                                                    int *__restrict__ bid, unsigned *__restrict__ bcnt)
 {
     __shared__ unsigned h[kWin];
     __shared__ unsigned long long ws[kWin + 1];
-    const int t = threadIdx.x, first = blockIdx.x * 256, i = first + t;
+    const int t = threadIdx.x, first = blockIdx.x * kWarmBodies;
     const int n = d.n;
     const Ctl *ctl = d.ctl;
     if (ctl->error) return;
     const int w0 = first / kBucketT + 1 - kWin / 4;
     if (t < kWin) h[t] = 0;
-    unsigned long long k = 0;
-    int b = 0;
-    if (i < n) {
-        b = order[i];
-        const float4 p = d.pm4[b];
-        float cx = ctl->root[0], cy = ctl->root[1], cz = ctl->root[2], r = ctl->radius;
+    int b[kWarmPer];
+    float4 p[kWarmPer];
+#pragma unroll
+    for (int j = 0; j < kWarmPer; ++j) {
+        const int ic = min(first + j * 256 + t, n - 1);
+        b[j] = order[ic];
+        p[j] = staged ? staged[ic] : d.pm4[b[j]];
+    }
+    load_window(spl, S, w0, ws);
+    const float rx = ctl->root[0], ry = ctl->root[1], rz = ctl->root[2], rr = ctl->radius;
+#pragma unroll
+    for (int j = 0; j < kWarmPer; ++j) {
+        const int i = first + j * 256 + t;
+        const bool valid = i < n;
+        unsigned long long k = 0;
+        float cx = rx, cy = ry, cz = rz, r = rr;
 #pragma unroll
         for (int L = 0; L < kKeyLevels; ++L) {
-            const int o = key_octant(cx, cy, cz, p);
+            const int o = key_octant(cx, cy, cz, p[j]);
             k = (k << 3) | (unsigned long long)o;
             key_descend(cx, cy, cz, r, o);
         }
-        key[i] = k;
-        val[i] = b;
-    }
-    load_window(spl, S, w0, ws);
-    if (i < n) {
-        int bk = bucket_in_window(ws, w0, k);
-        if (bk >= 0) {
-            atomicAdd(&h[bk - w0], 1u);
-        } else {
+        int bk = valid ? bucket_in_window(ws, w0, k) : -1;
+        const bool inwin = bk >= 0;
+        if (valid && !inwin) {
             bk = bucket_of(spl, S, k, i / kBucketT + 1);
             atomicAdd(&bcnt[bk], 1u);
         }
-        bid[i] = bk;
+        (void)claim_bin(h, inwin ? bk - w0 : 0, inwin);
+        if (valid) {
+            key[i] = k;
+            val[i] = b[j];
+            bid[i] = bk;
+        }
     }
     __syncthreads();
     if (t < kWin && h[t]) atomicAdd(&bcnt[w0 + t], h[t]);
@@ -461,9 +500,11 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(const Ctl *ctl, const un
     unsigned at[kWinPer];
 #pragma unroll
     for (int j = 0; j < kWinPer; ++j) {
-        if (first + j * 256 + t >= n) continue;
+        const bool valid = first + j * 256 + t < n;
         const unsigned o = (unsigned)(b[j] - w0);
-        at[j] = o < (unsigned)kWin ? atomicAdd(&h[o], 1u) : bstart[b[j]] + atomicAdd(&bfill[b[j]], 1u);
+        const bool inwin = valid && o < (unsigned)kWin;
+        at[j] = claim_bin(h, inwin ? (int)o : 0, inwin);
+        if (valid && !inwin) at[j] = bstart[b[j]] + atomicAdd(&bfill[b[j]], 1u);
     }
     __syncthreads();
     if (t < kWin) hb[t] = h[t] ? bstart[w0 + t] + atomicAdd(&bfill[w0 + t], h[t]) : 0u;
@@ -1154,21 +1195,18 @@ __global__ __launch_bounds__(256) void k_link(Dev d, KeyTree kt)
 }
 
 // Stage 4 (sort.cl:12-76) on the key-sorted tree: the octant-path order is the
-// build's sorted body order, so sorted[] is a copy of it and bnode[] gathers
-// (x, y, z, m) in that order.
+// build's sorted body order, so sorted[] is a copy of it.
 __global__ __launch_bounds__(256) void k_sort_keys(Dev d, const int *__restrict__ val)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= d.n || d.ctl->error) return;
-    const int b = val[g];
-    d.sorted[g] = b;
-    d.bnode[g] = d.pm4[b];
+    d.sorted[g] = val[g];
 }
 
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-hipError_t launch_build_keys(const Dev &d, bool warm, hipStream_t st)
+hipError_t launch_build_keys(const Dev &d, bool warm, bool staged, hipStream_t st)
 {
     const int n = d.n;
     const dim3 b256(256), g256((n + 255) / 256);
@@ -1183,7 +1221,8 @@ hipError_t launch_build_keys(const Dev &d, bool warm, hipStream_t st)
         const dim3 gw((n + kWinBodies - 1) / kWinBodies);
         hipLaunchKernelGGL(k_splitters, dim3((S + 1 + 255) / 256), b256, 0, st, d.ctl, ka, S, d.kb_spl, d.kb_bcnt,
                            d.kb_bfill, d.kb_big);
-        hipLaunchKernelGGL(k_keys_warm, g256, b256, 0, st, d, d.sorted, d.kb_spl, S, kb, vb, d.kb_lo, d.kb_bcnt);
+        hipLaunchKernelGGL(k_keys_warm, dim3((n + kWarmBodies - 1) / kWarmBodies), b256, 0, st, d, d.sorted,
+                           staged ? d.bnode : nullptr, d.kb_spl, S, kb, vb, d.kb_lo, d.kb_bcnt);
         hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, d.ctl, d.kb_bcnt, S, d.kb_bstart);
         hipLaunchKernelGGL(k_bucket_scatter, gw, b256, 0, st, d.ctl, kb, vb, d.kb_lo, ka, va, n, d.kb_bstart,
                            d.kb_bfill);

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     if (__ballot(valid) == 0ull) return; // wave-uniform
     // a lane past the last body sits at infinity: its d^2 is infinite, so it
     // accepts every cell and never holds up the vote (its sums are discarded)
-    const float4 me = valid ? d.bnode[pos] : make_float4(kFar, kFar, kFar, 0.f);
+    const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
     // alpha^2 |a_old|^2 of the relative test; +inf switches it off (no

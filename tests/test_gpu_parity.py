@@ -126,18 +126,25 @@ def _collapse(n=32768, seed=12):
     return tuple(np.ascontiguousarray(a, np.float32) for a in (p[:, 0], p[:, 1], p[:, 2], v[:, 0], v[:, 1], v[:, 2], m))
 
 
-@pytest.mark.parametrize("case,dt,large", [("plummer_32k", 0.0, False), ("plummer_32k", 1e-3, False),
-                                           ("collapse_32k", 1.0, True)])
-def test_warm_build_tracks_oracle(nb, oracle, case, dt, large):
+@pytest.mark.parametrize("case,dt,large,fused", [("plummer_32k", 0.0, False, False),
+                                                 ("plummer_32k", 1e-3, False, False),
+                                                 ("plummer_32k", 1e-3, False, True),
+                                                 ("collapse_32k", 1.0, True, False),
+                                                 ("collapse_32k", 1.0, True, True)])
+def test_warm_build_tracks_oracle(nb, oracle, case, dt, large, fused):
     """After a step the build starts from that tree's body order (sorted[]):
     bucketed by splitters taken from that tree's sorted keys, each bucket
     sorted on chip (or through device memory when it is large). The second
     tree against the oracle's tree of the same positions: membership, cells
     and sorted[] bit for bit. dt 0 leaves the positions (and the order)
     unchanged; in the collapse case nearly every body moves into a ball that
-    one bucket of the last tree covers, which is then far too large for LDS."""
+    one bucket of the last tree covers, which is then far too large for LDS.
+    With the fused update the build reads the positions from the records that
+    update wrote in the last order instead of gathering them."""
+    from nbody_amd import _lib
+
     st = _collapse() if case == "collapse_32k" else _universes()[case]()
-    sim = make_sim(st, theta=0.5, softening=0.05, dt=dt)
+    sim = make_sim(st, theta=0.5, softening=0.05, dt=dt, flags=_lib.NB_FUSED_UPDATE if fused else 0)
     sim.steps(1)
     x, y, z, vx, vy, vz = sim.state()
     orc = make_oracle((x, y, z, vx, vy, vz, st[6]), 0.5, 0.05, dt, 16)
