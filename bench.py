@@ -163,7 +163,9 @@ def stage_rooflines(stages, n, cells, fused, learned):
     stage in the reference's FP32 layout (4-byte words; C = live cells): bbox
     12N, build 16N + 52C, summarize 16N + 68C, sort 4N + 44C, integrate 96N
     with FP64 positions. The stage times are the HIP-event times of the whole
-    stage (every launch of it and the gaps between them)."""
+    stage (every launch of it and the gaps between them). The sort stage
+    launches nothing: the build's sorted body order is the octant-path order,
+    and its buffer becomes sorted[]."""
     algo = {"bbox": 12 * n, "build": 16 * n + 52 * cells, "summarize": 16 * n + 68 * cells,
             "sort": 4 * n + 44 * cells, "integrate": 96 * n}
     out = {}
@@ -171,6 +173,10 @@ def stage_rooflines(stages, n, cells, fused, learned):
         ms = stages[k]
         if k == "integrate" and fused:
             out[k] = {"algorithmic_bytes": b, "note": "fused into the force kernel's epilogue"}
+            continue
+        if k == "sort":
+            out[k] = {"algorithmic_bytes": b, "ms": ms,
+                      "note": "no launch: the build's sorted order buffer becomes sorted[]"}
             continue
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
         out[k] = {"algorithmic_bytes": b, "ms": ms, "GB/s": round(gbs, 1) if gbs else None,

@@ -75,6 +75,7 @@ struct nb_sim {
     bool ord_ready = false; // d.gorder holds an order once ord_out has passed
     bool keys_warm = false; // d.sorted holds the last tree's body order (the build starts from it)
     bool staged = false;    // d.bnode holds pm4 in d.sorted order (the last fused update wrote it)
+    bool sort_pending = false; // d.kb_val[0] holds the order of a build the sort stage has not taken yet
     // copyVertices staging: 2 N float4 (positions, then velocities), allocated
     // by the first nb_copy_vertices call and kept until nb_destroy
     float4 *vtx = nullptr;
@@ -308,7 +309,10 @@ int launch_stage(nb_sim *s, int stage)
     const int fused = (s->p.flags & NB_FUSED_UPDATE) ? 1 : 0;
     switch (stage) {
     case NB_STAGE_BBOX: e = nb::launch_bbox(d, s->bbox_blocks, s->stream); break;
-    case NB_STAGE_BUILD: e = nb::launch_build_keys(d, s->keys_warm, s->keys_warm && s->staged, s->stream); break;
+    case NB_STAGE_BUILD:
+        e = nb::launch_build_keys(d, s->keys_warm, s->keys_warm && s->staged, s->stream);
+        s->sort_pending = e == hipSuccess;
+        break;
     case NB_STAGE_SUMMARIZE: {
         int r = prepare_levels(s);
         if (r) return r;
@@ -319,7 +323,14 @@ int launch_stage(nb_sim *s, int stage)
         break;
     }
     case NB_STAGE_SORT:
-        e = nb::launch_sort_keys(d, s->stream);
+        // the build's sorted body order is the octant-path order (sort.cl:12-76):
+        // its buffer becomes sorted[], and the last order's buffer takes its
+        // place for the next build (launches already queued keep the pointers
+        // they were given)
+        if (s->sort_pending) {
+            std::swap(s->d.sorted, s->d.kb_val[0]);
+            s->sort_pending = false;
+        }
         s->keys_warm = true; // sorted[] holds this tree's body order for the next build
         s->staged = false;   // until the fused update writes bnode[] in that order
         break;
@@ -859,6 +870,7 @@ int nb_set_state(nb_sim *s, const float *x, const float *y, const float *z, cons
     s->ord_ready = false;  // the launch order of the coarse traversal belongs to the old bodies
     s->keys_warm = false;  // and so does sorted[]: the next build sorts from scratch
     s->staged = false;
+    s->sort_pending = false;
     reset_control(s);
     s->have_state = true;
     s->err.clear();

@@ -27,7 +27,9 @@
 //   levels   lvl_list : int32[cap]  live cells grouped by level
 //   sorted   bnode  : float4[N]     (x, y, z, m) in sorted order as the fused update
 //                                   wrote them (the next build's positions)
-//            sorted : int32[N]      body index at each sorted position
+//            sorted : int32[N]      body index at each sorted position: the
+//                                   build's sorted kb_val[0], whose buffer the
+//                                   sort stage swaps with this one
 //   build    kb_key, kb_val : uint64[N] and int32[N], two of each: octant-path
 //                                   keys and body indices through the radix
 //                                   passes, the sorted pair in [0]
@@ -113,7 +115,7 @@ struct Dev {
     double *cq64;
     float4 *cqa;
     float4 *cqb;
-    float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^2) of the child's extent b
+    float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^(2/5)) of the child's extent b
     int *lvl_list;
     unsigned char *clvl; // cap: each cell's level, or kNotLive (level histogram -> scatter)
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth
@@ -146,12 +148,10 @@ hipError_t launch_bbox(const Dev &d, int blocks, hipStream_t st);
 // warm: start from sorted[] and the last sorted keys; staged: bnode[] holds
 // the records in sorted[] order
 hipError_t launch_build_keys(const Dev &d, bool warm, bool staged, hipStream_t st);
-// stage 4 after the key-sorted build: sorted[] from its body order
-hipError_t launch_sort_keys(const Dev &d, hipStream_t st);
 constexpr int kKeyTile = 2048;     // keys per radix-sort tile (first build)
 constexpr int kKeyScanTile = 1024; // cell counts per scan tile
-constexpr int kBucketT = 512;      // warm builds: one splitter per kBucketT bodies
-constexpr int kBucketCap = 1024;   // buckets sorted by one wave up to this size
+constexpr int kBucketT = 256;      // warm builds: one splitter per kBucketT bodies
+constexpr int kBucketCap = 512;    // buckets sorted by one wave up to this size
 inline int bucket_samples(int n) { return (n + kBucketT - 1) / kBucketT; }
 hipError_t launch_levels(const Dev &d, hipStream_t st); // group live cells by level
 // Summarize and sort read each level's cell count on the device; the host's

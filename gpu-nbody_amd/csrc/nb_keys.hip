@@ -441,36 +441,68 @@ __global__ __launch_bounds__(256) void k_keys_warm(Dev d, const int *__restrict_
     if (t < kWin && h[t]) atomicAdd(&bcnt[w0 + t], h[t]);
 }
 
-// exclusive scan of the S + 1 bucket counts: bstart[0 .. S+1] (one block)
+// exclusive scan of the S + 1 bucket counts: bstart[0 .. S+1] (one block).
+// Chunks of kBsChunk counts go through LDS (coalesced loads and stores; the
+// 8 of a thread padded to 9 words, so its reads hit distinct banks), each
+// thread scans 8 consecutive counts, and the chunk's carry moves on.
+constexpr int kBsPer = 8;
+constexpr int kBsChunk = 1024 * kBsPer;
+
+__device__ __forceinline__ int bs_pad(int i) { return i + i / kBsPer; }
+
 __global__ __launch_bounds__(1024) void k_bucket_scan(const Ctl *ctl, const unsigned *__restrict__ bcnt, int S,
                                                       unsigned *__restrict__ bstart)
 {
-    __shared__ unsigned s[1024];
-    const int t = threadIdx.x;
+    __shared__ unsigned c[kBsChunk + kBsChunk / kBsPer];
+    __shared__ unsigned wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (ctl->error) return;
-    const int nb = S + 1, per = (nb + 1023) / 1024;
-    unsigned sum = 0;
-    for (int k = 0; k < per; ++k) {
-        const int j = t * per + k;
-        if (j < nb) sum += bcnt[j];
-    }
-    s[t] = sum;
-    __syncthreads();
-    for (int st = 1; st < 1024; st <<= 1) {
-        const unsigned u = t >= st ? s[t - st] : 0u;
-        __syncthreads();
-        s[t] += u;
-        __syncthreads();
-    }
-    unsigned run = s[t] - sum;
-    for (int k = 0; k < per; ++k) {
-        const int j = t * per + k;
-        if (j < nb) {
-            bstart[j] = run;
-            run += bcnt[j];
+    const int nb = S + 1;
+    unsigned carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += kBsChunk) {
+#pragma unroll
+        for (int k = 0; k < kBsPer; ++k) {
+            const int i = k * 1024 + t;
+            c[bs_pad(i)] = c0 + i < nb ? bcnt[c0 + i] : 0u;
         }
+        __syncthreads();
+        unsigned v[kBsPer], sum = 0;
+#pragma unroll
+        for (int k = 0; k < kBsPer; ++k) {
+            v[k] = c[bs_pad(t * kBsPer + k)];
+            sum += v[k];
+        }
+        unsigned x = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        unsigned below = 0, total = 0;
+#pragma unroll
+        for (int ww = 0; ww < 16; ++ww) {
+            const unsigned ws = wsum[ww];
+            below += ww < w ? ws : 0u;
+            total += ws;
+        }
+        unsigned run = carry + below + x - sum;
+#pragma unroll
+        for (int k = 0; k < kBsPer; ++k) {
+            c[bs_pad(t * kBsPer + k)] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kBsPer; ++k) {
+            const int i = k * 1024 + t;
+            if (c0 + i < nb) bstart[c0 + i] = c[bs_pad(i)];
+        }
+        carry += total;
+        __syncthreads();
     }
-    if (t == 1023) bstart[nb] = s[1023];
+    if (t == 0) bstart[nb] = carry;
 }
 
 // Each body takes the next place of its bucket: LDS counters for the buckets
@@ -903,12 +935,12 @@ __global__ __launch_bounds__(256) void k_deep_runs(Dev d, const unsigned long lo
     }
 }
 
-// Common prefix of sorted keys i and j in bits (3 per level); -1 outside.
-__device__ __forceinline__ int key_delta(const unsigned long long *__restrict__ key,
-                                         const unsigned long long *__restrict__ deep, int n, int i, int j)
+// Common prefix in bits (3 per level) of sorted bodies i and j, whose keys are
+// ki and kj.
+__device__ __forceinline__ int delta_of(unsigned long long ki, unsigned long long kj,
+                                        const unsigned long long *__restrict__ deep, int n, int i, int j)
 {
-    if (j < 0 || j >= n) return -1;
-    const unsigned long long x = key[i] ^ key[j];
+    const unsigned long long x = ki ^ kj;
     if (x) return __clzll((long long)x) - 1;
     // equal 21-level keys: i and j are in one run, whose deep words are set
 #pragma unroll
@@ -917,6 +949,14 @@ __device__ __forceinline__ int key_delta(const unsigned long long *__restrict__ 
         if (y) return 63 * (w + 1) + __clzll((long long)y) - 1;
     }
     return 63 * (kDeepWords + 1);
+}
+
+// Common prefix of sorted keys i and j in bits (3 per level); -1 outside.
+__device__ __forceinline__ int key_delta(const unsigned long long *__restrict__ key,
+                                         const unsigned long long *__restrict__ deep, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    return delta_of(key[i], key[j], deep, n, i, j);
 }
 
 // octant of sorted body `pos` at `level`
@@ -936,46 +976,115 @@ __device__ __forceinline__ int key_digit(const unsigned long long *__restrict__ 
 // delta(i)]. delta(parent) is the larger of the prefixes the range shares with
 // its two outside neighbours (the parent extends the range towards the
 // neighbour that shares more; -1 for the root).
+//
+// The searches of a block's nodes read the keys from a window in LDS (the
+// block's own keys and kBtHalo on each side, loaded once, coalesced); only a
+// probe past the window (a node whose range reaches further) reads the key
+// from device memory. Most ranges are short, so the searches' dependent loads
+// become LDS reads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bintree(Dev d, const unsigned long long *__restrict__ key,
-                                                 const unsigned long long *__restrict__ deep, int *__restrict__ iparent,
-                                                 int *__restrict__ lparent, int *__restrict__ lo_out,
-                                                 int *__restrict__ own, int *__restrict__ ktop)
+constexpr int kBtThreads = 512;
+constexpr int kBtHalo = 512;
+constexpr int kBtSpan = kBtThreads + 2 * kBtHalo;
+
+// keys through the block's LDS window where it has them, else from device memory
+struct KeysAny {
+    const unsigned long long *ks;
+    const unsigned long long *__restrict__ key;
+    int base;
+    __device__ __forceinline__ unsigned long long operator()(int j) const
+    {
+        const unsigned t = (unsigned)(j - base);
+        return t < (unsigned)kBtSpan ? ks[t] : key[j];
+    }
+};
+
+// delta(i, j) > dd for i's key ki, with lim = limit(dd). Keys use 63 bits, so
+// delta = clz(ki ^ kj) - 1 below 63, and delta > dd (dd < 63) holds exactly
+// when ki ^ kj < 2^(62 - dd) (equal keys included: their delta is >= 63); one
+// 64-bit compare instead of the count of leading zeros. A dd of 63 or more (i
+// inside a run of equal keys) compares the deep words.
+__device__ __forceinline__ unsigned long long delta_limit(int dd) { return dd < 63 ? 1ull << (62 - dd) : 0ull; }
+
+template <class K>
+__device__ __forceinline__ bool delta_above(const K &ka, const unsigned long long *__restrict__ deep, int n, int i,
+                                            unsigned long long ki, int j, int dd, unsigned long long lim)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = d.n;
-    if (i >= n - 1 || d.ctl->error) return;
-    const int dir = key_delta(key, deep, n, i, i + 1) - key_delta(key, deep, n, i, i - 1) > 0 ? 1 : -1;
-    const int dmin = key_delta(key, deep, n, i, i - dir);
-    long long lmax = 2;
-    while (key_delta(key, deep, n, i, (int)(i + lmax * dir)) > dmin) lmax <<= 1;
-    long long l = 0;
-    for (long long t = lmax >> 1; t >= 1; t >>= 1)
-        if (key_delta(key, deep, n, i, (int)(i + (l + t) * dir)) > dmin) l += t;
-    const int j = (int)(i + l * dir);
-    const int dn = key_delta(key, deep, n, i, j);
-    long long s = 0, t = l;
+    if (j < 0 || j >= n) return false;
+    if (lim) return (ki ^ ka(j)) < lim;
+    return delta_of(ki, ka(j), deep, n, i, j) > dd;
+}
+
+template <class K>
+__device__ __forceinline__ int delta_at(const K &ka, const unsigned long long *__restrict__ deep, int n, int i,
+                                        unsigned long long ki, int j)
+{
+    return j < 0 || j >= n ? -1 : delta_of(ki, ka(j), deep, n, i, j);
+}
+
+struct BinOut {
+    int *__restrict__ iparent, *__restrict__ lparent, *__restrict__ lo, *__restrict__ own, *__restrict__ ktop;
+};
+
+// Internal node i: its searches, then its outputs. dir and dmin come from
+// the neighbours i - 1 and i + 1. n <= 2^26 (the pool limit), so every probe
+// fits an int.
+template <class K>
+__device__ __forceinline__ void bintree_node(const Dev &d, const K &ka, const unsigned long long *__restrict__ deep,
+                                             int n, int i, const BinOut &o)
+{
+    const unsigned long long ki = ka(i);
+    const int dir = delta_at(ka, deep, n, i, ki, i + 1) - delta_at(ka, deep, n, i, ki, i - 1) > 0 ? 1 : -1;
+    const int dmin = delta_at(ka, deep, n, i, ki, i - dir);
+    const unsigned long long lmin = delta_limit(dmin);
+    int lmax = 2;
+    while (delta_above(ka, deep, n, i, ki, i + lmax * dir, dmin, lmin)) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta_above(ka, deep, n, i, ki, i + (l + t) * dir, dmin, lmin)) l += t;
+    const int j = i + l * dir;
+    const int dn = delta_at(ka, deep, n, i, ki, j);
+    const unsigned long long ln = delta_limit(dn);
+    int s = 0, t = l;
     do {
         t = (t + 1) >> 1;
-        if (key_delta(key, deep, n, i, (int)(i + (s + t) * dir)) > dn) s += t;
+        if (delta_above(ka, deep, n, i, ki, i + (s + t) * dir, dn, ln)) s += t;
     } while (t > 1);
-    const int g = (int)(i + s * dir + min(dir, 0));
+    const int g = i + s * dir + min(dir, 0);
     const int lo = min(i, j), hi = max(i, j);
-    if (lo == g) lparent[g] = i;
-    else iparent[g] = i;
-    if (hi == g + 1) lparent[g + 1] = i;
-    else iparent[g + 1] = i;
-    lo_out[i] = lo;
-    if (i == 0) iparent[0] = -1;
-    const int dp = max(key_delta(key, deep, n, lo, lo - 1), key_delta(key, deep, n, hi, hi + 1));
+    if (lo == g) o.lparent[g] = i;
+    else o.iparent[g] = i;
+    if (hi == g + 1) o.lparent[g + 1] = i;
+    else o.iparent[g + 1] = i;
+    o.lo[i] = lo;
+    if (i == 0) o.iparent[0] = -1;
+    const int dp = max(delta_at(ka, deep, n, lo, ka(lo), lo - 1), delta_at(ka, deep, n, hi, ka(hi), hi + 1));
     const int kt = dp < 0 ? 0 : dp / 3 + 1;
     int kb = dn / 3;
     if (kb >= kBuildLevelLimit) {
         flag_error(d.ctl, kErrDepth);
         kb = kBuildLevelLimit - 1;
     }
-    own[i] = max(0, kb - kt + 1);
-    ktop[i] = kt;
+    o.own[i] = max(0, kb - kt + 1);
+    o.ktop[i] = kt;
+}
+
+__global__ __launch_bounds__(kBtThreads) void k_bintree(Dev d, const unsigned long long *__restrict__ key,
+                                                        const unsigned long long *__restrict__ deep, BinOut o)
+{
+    __shared__ unsigned long long ks[kBtSpan];
+    const int n = d.n;
+    Ctl *ctl = d.ctl;
+    if (ctl->error) return; // block-uniform
+    const int base = (int)blockIdx.x * kBtThreads - kBtHalo;
+    for (int t = threadIdx.x; t < kBtSpan; t += kBtThreads) {
+        const int g = base + t;
+        ks[t] = g >= 0 && g < n ? key[g] : 0ull;
+    }
+    __syncthreads();
+    const int i = (int)blockIdx.x * kBtThreads + (int)threadIdx.x;
+    if (i >= n - 1) return;
+    bintree_node(d, KeysAny{ks, key, base}, deep, n, i, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,11 +1149,12 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const Ctl *ctl, const int *_
 
 // one block: exclusive scan of the tile sums in place (each thread a run of
 // them); the tiles' depths into the running maximum
-__global__ __launch_bounds__(1024) void k_scan_sums(Ctl *ctl, int *__restrict__ sums, int ntiles,
+__global__ __launch_bounds__(1024) void k_scan_sums(Dev d, int *__restrict__ sums, int ntiles,
                                                    const int *__restrict__ tdepth)
 {
     __shared__ int s[1024];
     __shared__ int wdep[16];
+    Ctl *ctl = d.ctl;
     if (ctl->error) return;
     const int t = threadIdx.x;
     const int per = (ntiles + 1023) / 1024;
@@ -1080,6 +1190,16 @@ __global__ __launch_bounds__(1024) void k_scan_sums(Ctl *ctl, int *__restrict__ 
         for (int k = 0; k < 16; ++k) dep = max(dep, wdep[k]);
         if (dep > 0) __hip_atomic_fetch_max(&ctl->max_depth, dep, __ATOMIC_RELAXED, AGENT);
     }
+    // the cell count (the total of the scan), the pool check, and the root's record
+    if (t == 1023) {
+        const int C = s[1023];
+        if (C > d.cap) {
+            flag_error(ctl, kErrPool);
+        } else {
+            ctl->n_cells = C;
+            d.cinfo[0] = make_int4(-1, 0, (int)ctl->epoch, 0);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,27 +1227,26 @@ __device__ __forceinline__ int holding_cell(const KeyTree &kt, int p)
     return -1;
 }
 
-// the cell count, the pool check, and the root's record
-__global__ void k_cell_count(Dev d, KeyTree kt)
+// one body: the root is the one cell (k_scan_sums counts the cells otherwise)
+__global__ void k_one_cell(Dev d)
 {
     Ctl *ctl = d.ctl;
     if (ctl->error) return;
-    const int n = d.n;
-    const int C = n >= 2 ? first_cell(kt, n - 2) + kt.own[n - 2] : 1;
-    if (C > d.cap) {
-        flag_error(ctl, kErrPool);
-        return;
-    }
-    ctl->n_cells = C;
+    ctl->n_cells = 1;
     d.cinfo[0] = make_int4(-1, 0, (int)ctl->epoch, 0);
 }
 
 // Position g: the cells of internal node g (g < n - 1) -- each in the slot of
 // its octant in its parent, and its record -- and sorted body g, in the
 // deepest cell of its nearest holding ancestor, at the level one below the
-// larger prefix it shares with a neighbour. The loads are issued in two
-// rounds of independent reads (clamped indices, no branch in between): the
-// kernel is bound by their latency, not by its arithmetic.
+// larger prefix it shares with a neighbour. The kernel is bound by the
+// latency of its dependent reads, not by its arithmetic: a thread takes
+// kLinkPer positions (kLinkPer * 256 apart in a block's span), and their
+// loads are issued together, in two rounds of independent reads (clamped
+// indices, no branch in between).
+constexpr int kLinkPer = 2;
+constexpr int kLinkSpan = 256 * kLinkPer;
+
 __device__ __forceinline__ int deepest_of(const KeyTree &kt, int own, int ob, int ts, int p)
 {
     // the holding cell of binary node p, given p's own, obase and tile sum:
@@ -1137,70 +1256,93 @@ __device__ __forceinline__ int deepest_of(const KeyTree &kt, int own, int ob, in
 
 __global__ __launch_bounds__(256) void k_link(Dev d, KeyTree kt)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = d.n;
     Ctl *ctl = d.ctl;
-    if (g >= n || ctl->error) return;
+    if (ctl->error) return;
     const uint32_t ep = ctl->epoch;
+    const int g0 = blockIdx.x * kLinkSpan + threadIdx.x;
     if (n < 2) { // the root holds the one body
-        d.slot[key_digit(kt.key, kt.deep, n, 0, 0)] = slot_tag(ep, kt.val[0]);
+        if (g0 == 0) d.slot[key_digit(kt.key, kt.deep, n, 0, 0)] = slot_tag(ep, kt.val[0]);
         return;
     }
-    // round 1: this position's node and leaf records and the keys around it
-    const int gi = min(g, n - 2);
-    const int own = kt.own[gi], top = kt.ktop[gi], lo = kt.lo[gi], ip = kt.iparent[gi];
-    const int ob = kt.obase[gi], ts = kt.tsum[gi / kScanTile];
-    const int lp = kt.lparent[g];
-    const int body = kt.val[g];
-    const unsigned long long kg = kt.key[g], km = kt.key[max(g - 1, 0)], kq = kt.key[min(g + 1, n - 1)];
-    // round 2: the parents' records, and the key of the node's first body
-    const int ipc = max(ip, 0);
-    const int own_ip = kt.own[ipc], ob_ip = kt.obase[ipc], ts_ip = kt.tsum[ipc / kScanTile];
-    const int own_lp = kt.own[lp], ob_lp = kt.obase[lp], ts_lp = kt.tsum[lp / kScanTile];
-    const unsigned long long klo = kt.key[lo];
-    if (g < n - 1 && own > 0) {
-        int par = -1;
-        if (top > 0) {
-            par = deepest_of(kt, own_ip, ob_ip, ts_ip, ip);
-            if (par < 0) {
-                flag_error(ctl, kErrSync);
-                return;
+    // round 1: each position's node and leaf records and the keys around it
+    int own[kLinkPer], top[kLinkPer], lo[kLinkPer], ip[kLinkPer], ob[kLinkPer], ts[kLinkPer], lp[kLinkPer];
+    int body[kLinkPer];
+    unsigned long long kg[kLinkPer], km[kLinkPer], kq[kLinkPer];
+#pragma unroll
+    for (int u = 0; u < kLinkPer; ++u) {
+        const int g = min(g0 + u * 256, n - 1);
+        const int gi = min(g, n - 2);
+        own[u] = kt.own[gi];
+        top[u] = kt.ktop[gi];
+        lo[u] = kt.lo[gi];
+        ip[u] = kt.iparent[gi];
+        ob[u] = kt.obase[gi];
+        ts[u] = kt.tsum[gi / kScanTile];
+        lp[u] = kt.lparent[g];
+        body[u] = kt.val[g];
+        kg[u] = kt.key[g];
+        km[u] = kt.key[max(g - 1, 0)];
+        kq[u] = kt.key[min(g + 1, n - 1)];
+    }
+    // round 2: the parents' records, and the key of each node's first body
+    int own_ip[kLinkPer], ob_ip[kLinkPer], ts_ip[kLinkPer], own_lp[kLinkPer], ob_lp[kLinkPer], ts_lp[kLinkPer];
+    unsigned long long klo[kLinkPer];
+#pragma unroll
+    for (int u = 0; u < kLinkPer; ++u) {
+        const int ipc = max(ip[u], 0);
+        own_ip[u] = kt.own[ipc];
+        ob_ip[u] = kt.obase[ipc];
+        ts_ip[u] = kt.tsum[ipc / kScanTile];
+        own_lp[u] = kt.own[lp[u]];
+        ob_lp[u] = kt.obase[lp[u]];
+        ts_lp[u] = kt.tsum[lp[u] / kScanTile];
+        klo[u] = kt.key[lo[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kLinkPer; ++u) {
+        const int g = g0 + u * 256;
+        if (g >= n) break;
+        if (g < n - 1 && own[u] > 0) {
+            int par = -1;
+            if (top[u] > 0) {
+                par = deepest_of(kt, own_ip[u], ob_ip[u], ts_ip[u], ip[u]);
+                if (par < 0) {
+                    flag_error(ctl, kErrSync);
+                    return;
+                }
+            }
+            const int base = ob[u] + ts[u];
+            for (int k = top[u]; k < top[u] + own[u]; ++k) {
+                const int cid = base + (k - top[u]);
+                if (k > 0) {
+                    const int L = k - 1;
+                    const int dg = L < kKeyLevels ? (int)((klo[u] >> (3 * (kKeyLevels - 1 - L))) & 7ull)
+                                                  : key_digit(kt.key, kt.deep, n, lo[u], L);
+                    d.slot[(size_t)par * 8 + dg] = slot_tag(ep, n + cid);
+                    d.cinfo[cid] = make_int4(par, k | (dg << 8), (int)ep, 0);
+                }
+                par = cid;
             }
         }
-        const int base = ob + ts;
-        for (int k = top; k < top + own; ++k) {
-            const int cid = base + (k - top);
-            if (k > 0) {
-                const int L = k - 1;
-                const int dg = L < kKeyLevels ? (int)((klo >> (3 * (kKeyLevels - 1 - L))) & 7ull)
-                                              : key_digit(kt.key, kt.deep, n, lo, L);
-                d.slot[(size_t)par * 8 + dg] = slot_tag(ep, n + cid);
-                d.cinfo[cid] = make_int4(par, k | (dg << 8), (int)ep, 0);
-            }
-            par = cid;
+        const int par = deepest_of(kt, own_lp[u], ob_lp[u], ts_lp[u], lp[u]);
+        if (par < 0) {
+            flag_error(ctl, kErrSync);
+            return;
         }
+        // common prefixes with the neighbours from the keys in hand (the deep
+        // words only when a key is repeated)
+        const int dm = g == 0 ? -1
+                              : (kg[u] != km[u] ? __clzll((long long)(kg[u] ^ km[u])) - 1
+                                                : key_delta(kt.key, kt.deep, n, g, g - 1));
+        const int dq = g == n - 1 ? -1
+                                  : (kg[u] != kq[u] ? __clzll((long long)(kg[u] ^ kq[u])) - 1
+                                                    : key_delta(kt.key, kt.deep, n, g, g + 1));
+        const int lv = max(dm, dq) / 3;
+        const int dg = lv < kKeyLevels ? (int)((kg[u] >> (3 * (kKeyLevels - 1 - lv))) & 7ull)
+                                       : key_digit(kt.key, kt.deep, n, g, lv);
+        d.slot[(size_t)par * 8 + dg] = slot_tag(ep, body[u]);
     }
-    const int par = deepest_of(kt, own_lp, ob_lp, ts_lp, lp);
-    if (par < 0) {
-        flag_error(ctl, kErrSync);
-        return;
-    }
-    // common prefixes with the neighbours from the keys in hand (the deep
-    // words only when a key is repeated)
-    const int dm = g == 0 ? -1 : (kg != km ? __clzll((long long)(kg ^ km)) - 1 : key_delta(kt.key, kt.deep, n, g, g - 1));
-    const int dq = g == n - 1 ? -1 : (kg != kq ? __clzll((long long)(kg ^ kq)) - 1 : key_delta(kt.key, kt.deep, n, g, g + 1));
-    const int lv = max(dm, dq) / 3;
-    const int dg = lv < kKeyLevels ? (int)((kg >> (3 * (kKeyLevels - 1 - lv))) & 7ull) : key_digit(kt.key, kt.deep, n, g, lv);
-    d.slot[(size_t)par * 8 + dg] = slot_tag(ep, body);
-}
-
-// Stage 4 (sort.cl:12-76) on the key-sorted tree: the octant-path order is the
-// build's sorted body order, so sorted[] is a copy of it.
-__global__ __launch_bounds__(256) void k_sort_keys(Dev d, const int *__restrict__ val)
-{
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= d.n || d.ctl->error) return;
-    d.sorted[g] = val[g];
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,22 +1393,17 @@ hipError_t launch_build_keys(const Dev &d, bool warm, bool staged, hipStream_t s
     hipLaunchKernelGGL(k_deep_runs, g256, b256, 0, st, d, ka, va, d.kb_deep, d.kb_ipar, d.kb_lpar, kb, vb);
     KeyTree kt{ka, d.kb_deep, va, d.kb_ipar, d.kb_lpar, d.kb_lo, d.kb_own, d.kb_ktop, d.kb_obase, d.kb_scan};
     if (n >= 2) {
-        const dim3 gi((n - 1 + 255) / 256);
-        hipLaunchKernelGGL(k_bintree, gi, b256, 0, st, d, ka, d.kb_deep, d.kb_ipar, d.kb_lpar, d.kb_lo, d.kb_own,
-                           d.kb_ktop);
+        const dim3 gi((n - 1 + kBtThreads - 1) / kBtThreads);
+        const BinOut bo{d.kb_ipar, d.kb_lpar, d.kb_lo, d.kb_own, d.kb_ktop};
+        hipLaunchKernelGGL(k_bintree, gi, dim3(kBtThreads), 0, st, d, ka, d.kb_deep, bo);
         const int m = n - 1, nt = (m + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, st, d.ctl, d.kb_own, d.kb_ktop, d.kb_obase, m,
                            d.kb_scan, d.kb_tdep);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, d.ctl, d.kb_scan, nt, d.kb_tdep);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, d, d.kb_scan, nt, d.kb_tdep);
+    } else {
+        hipLaunchKernelGGL(k_one_cell, dim3(1), dim3(1), 0, st, d);
     }
-    hipLaunchKernelGGL(k_cell_count, dim3(1), dim3(1), 0, st, d, kt);
-    hipLaunchKernelGGL(k_link, g256, b256, 0, st, d, kt);
-    return hipGetLastError();
-}
-
-hipError_t launch_sort_keys(const Dev &d, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_sort_keys, dim3((d.n + 255) / 256), dim3(256), 0, st, d, d.kb_val[0]);
+    hipLaunchKernelGGL(k_link, dim3((n + kLinkSpan - 1) / kLinkSpan), b256, 0, st, d, kt);
     return hipGetLastError();
 }
 

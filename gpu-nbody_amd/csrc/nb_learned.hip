@@ -6,9 +6,12 @@
 // largest distance of its bodies from its centre of mass, bounded in
 // summarize) and mass m at s^2 = |d|^2 + eps^2 when
 //     s^2 >= b^2 / theta_c^2 + eps^2           (extent test: |d| >= b / theta_c)
-//     (m b^3)^2 <= alpha^2 |a_old|^2 s^10      (m b^3 / s^5, the size of the
-//                                               first term left out, within
-//                                               alpha |a_old|)
+//     s^2 >= (m b^3)^(2/5) (alpha^2 |a_old|^2)^(-1/5)
+//                                              (relative test: m b^3 / s^5, the
+//                                               size of the first term left
+//                                               out, within alpha |a_old|; its
+//                                               two factors are rounded to FP32
+//                                               once, per cell and per body)
 // with a_old the body's acceleration of the previous step (the relative test
 // is off for a lane while there is none). Every accepted cell contributes its
 // monopole and its quadrupole under the softened kernel: with d = centre of
@@ -194,6 +197,12 @@ __device__ __forceinline__ void mlp_wave(const uint16_t *fs, const Mlp &m, int l
 // ---------------------------------------------------------------------------
 // Coarse traversal
 // ---------------------------------------------------------------------------
+// lane k's value of x (k wave-uniform)
+__device__ __forceinline__ float lane_read(float x, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
 template <int OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
                                                   const float4 *__restrict__ cdata,
@@ -207,6 +216,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
 {
     constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
     constexpr bool kGeo = OUT == kOutAccel; // the fine traversal: the force stage's opening table
+    constexpr bool kLaneQ = true; // quadrupole records by vector loads and lane reads
     // a wave's traversal stack; after the traversal the learned step reuses it
     // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
     __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
@@ -240,13 +250,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
-    // alpha^2 |a_old|^2 of the relative test; +inf switches it off (no
-    // previous acceleration yet, as in the force stage's kick, or alpha = 0)
-    float aa = __builtin_inff();
+    // the body's factor (alpha^2 |a_old|^2)^(-1/5) of the relative test's
+    // threshold; 0 switches it off (no previous acceleration yet, as in the
+    // force stage's kick, or alpha = 0)
+    float a5 = 0.f;
     if (!kGeo && valid && step > 0 && d.alpha2_coarse > 0.f) {
         const float4 a0 = d.acc4[d.sorted[pos]];
         const float a2 = fmaf(a0.x, a0.x, fmaf(a0.y, a0.y, a0.z * a0.z));
-        if (a2 > 0.f) aa = d.alpha2_coarse * a2;
+        if (a2 > 0.f) a5 = (float)pow((double)(d.alpha2_coarse * a2), -0.2);
     }
 
     if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
@@ -284,13 +295,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             q[k] = cdp[k];
             if (!kGeo) mc[k] = cmac[(size_t)c * 8 + k];
         }
+        // the children's quadrupole records, child k in lanes k, k + 8, ...:
+        // two vector loads issued now, waited for at the first accepted cell
+        // (which takes its record by lane reads), so their latency hides
+        // behind the decisions and the bodies
+        float4 qav, qbv;
+        if constexpr (kLaneQ) {
+            qav = cqa[(size_t)c * 8 + (lane & 7)];
+            qbv = cqb[(size_t)c * 8 + (lane & 7)];
+        }
         // First the opening decisions, wave-uniform bit masks of the children
         // (children are compacted to the front: past the first negative entry
         // every one is negative). A cell: the opening test of this traversal.
         // The coarse one takes the extent test first (threshold b^2 /
-        // theta_c^2 + eps^2 from summarize) and works out the relative test,
-        // with the fifth power of d2, only when every lane passed it: both
-        // votes are wave-uniform, so the short circuit is a scalar branch.
+        // theta_c^2 + eps^2 from summarize) and the relative test (threshold
+        // the product of the cell's and the body's factors) only when every
+        // lane passed it: both votes are wave-uniform, so the short circuit is
+        // a scalar branch.
         // Lane masks come straight from the compares (3 = ordered >=, 5 =
         // ordered <=). An opened cell's stack entry goes into lane `npush` of
         // `push` (a lane write, no branch on the lane); all of them are stored
@@ -311,12 +332,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             const uint64_t exec = __builtin_amdgcn_read_exec();
             bool take = __builtin_amdgcn_fcmpf(d2, kGeo ? thr_geo : mc[k].x, 3) == exec;
-            if (!kGeo && take) {
-                float p = d2 * d2;
-                p = p * p;
-                p = p * d2;
-                take = __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5) == exec;
-            }
+            if (!kGeo && take) take = __builtin_amdgcn_fcmpf(mc[k].y * a5, d2, 5) == exec;
             if (take) {
                 cells |= 1u << k;
             } else {
@@ -329,11 +345,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         // around its own work, so the running sums are updated in place (a
         // branch that both accumulates and pushes makes the compiler copy
         // them at every child): bodies, cell monopoles and cell quadrupoles
-        // summed separately (the features need the parts), then one
-        // compensated add. d is worked out again, with the same operations as
-        // for the test.
-        float pbx = 0.f, pby = 0.f, pbz = 0.f, pmx = 0.f, pmy = 0.f, pmz = 0.f;
-        float pqx = 0.f, pqy = 0.f, pqz = 0.f;
+        // summed separately (the features need the parts). The learned step
+        // adds its terms to its three running sums directly; the fine
+        // traversal sums a popped cell's terms from zero for its compensated
+        // add, and the coarse one without features (one sum of monopole and
+        // quadrupole) from zero for its running sums, which keeps it within
+        // 1e-5 of the oracle's FP64 sums. d is worked out again, with the
+        // same operations as for the test.
+        float pbx, pby, pbz, pmx, pmy, pmz, pqx, pqy, pqz;
+        if constexpr (kGeo || !kFeatOn) {
+            pbx = pby = pbz = pmx = pmy = pmz = pqx = pqy = pqz = 0.f;
+        } else {
+            pbx = bx;
+            pby = by;
+            pbz = bz;
+            pmx = sx;
+            pmy = sy;
+            pmz = sz;
+            pqx = qx;
+            pqy = qy;
+            pqz = qz;
+        }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (!((bodies >> k) & 1u)) continue;
@@ -353,8 +385,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float4 qk = q[k];
             const float dx = qk.x - me.x, dy = qk.y - me.y, dz = qk.z - me.z;
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-            const float4 qa = cqa[(size_t)c * 8 + k]; // (Qxx, Qyy, Qzz, T eps^2)
-            const float4 qb = cqb[(size_t)c * 8 + k]; // (Qxy, Qxz, Qyz, 0)
+            float4 qa, qb; // (Qxx, Qyy, Qzz, T eps^2), (Qxy, Qxz, Qyz, 0)
+            if constexpr (kLaneQ) {
+                qa = make_float4(lane_read(qav.x, k), lane_read(qav.y, k), lane_read(qav.z, k), lane_read(qav.w, k));
+                qb = make_float4(lane_read(qbv.x, k), lane_read(qbv.y, k), lane_read(qbv.z, k), 0.f);
+            } else {
+                qa = cqa[(size_t)c * 8 + k];
+                qb = cqb[(size_t)c * 8 + k];
+            }
             const float ri = __builtin_amdgcn_rsqf(d2);
             const float r2i = ri * ri;
             const float r3 = r2i * ri;
@@ -403,19 +441,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             kahan_add(sx, kx, (pbx + pmx) + pqx);
             kahan_add(sy, ky, (pby + pmy) + pqy);
             kahan_add(sz, kz, (pbz + pmz) + pqz);
-        } else {
-            // the coarse force: its parts summed apart (the features need them)
+        } else if constexpr (!kFeatOn) {
             sx += pmx;
             sy += pmy;
             sz += pmz;
             bx += pbx;
             by += pby;
             bz += pbz;
-            if (kFeatOn) {
-                qx += pqx;
-                qy += pqy;
-                qz += pqz;
-            }
+        } else {
+            // the coarse force: its parts summed apart (the features need them)
+            sx = pmx;
+            sy = pmy;
+            sz = pmz;
+            bx = pbx;
+            by = pby;
+            bz = pbz;
+            qx = pqx;
+            qy = pqy;
+            qz = pqz;
         }
         ninter += cnt;
         if (bad) break;
