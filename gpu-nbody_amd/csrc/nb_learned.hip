@@ -6,12 +6,9 @@
 // largest distance of its bodies from its centre of mass, bounded in
 // summarize) and mass m at s^2 = |d|^2 + eps^2 when
 //     s^2 >= b^2 / theta_c^2 + eps^2           (extent test: |d| >= b / theta_c)
-//     s^2 >= (m b^3)^(2/5) (alpha^2 |a_old|^2)^(-1/5)
-//                                              (relative test: m b^3 / s^5, the
-//                                               size of the first term left
-//                                               out, within alpha |a_old|; its
-//                                               two factors are rounded to FP32
-//                                               once, per cell and per body)
+//     (m b^3)^2 <= alpha^2 |a_old|^2 s^10      (m b^3 / s^5, the size of the
+//                                               first term left out, within
+//                                               alpha |a_old|)
 // with a_old the body's acceleration of the previous step (the relative test
 // is off for a lane while there is none). Every accepted cell contributes its
 // monopole and its quadrupole under the softened kernel: with d = centre of
@@ -161,27 +158,37 @@ __device__ __forceinline__ void mlp_wave(const uint16_t *fs, const Mlp &m, int l
     for (int cb = 0; cb < 4; ++cb) {
         const bf16x8 xb = g < 3 ? load_bf16x8(fs + (cb * 16 + i16) * kFeat + 8 * g)
                                 : __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
-        f32x4 d1[4];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-            d1[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w1f + (rb * 64 + lane) * 8), xb, z4, 0, 0, 0);
+        // each layer in two halves of 32 hidden rows (two 16-row blocks),
+        // each half's activations packed before the next half starts: fewer
+        // accumulators live at once
         const float4 *b1 = reinterpret_cast<const float4 *>(m.b1 + 4 * g);
-        const bf16x8 h1a = pack_gelu(d1[0], b1[0], d1[1], b1[4]);
-        const bf16x8 h1b = pack_gelu(d1[2], b1[8], d1[3], b1[12]);
-        // keep each layer's weight loads next to their use (register pressure)
-        __builtin_amdgcn_sched_barrier(0);
-        f32x4 d2[4];
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            d2[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 0) * 64 + lane) * 8), h1a,
-                                                             z4, 0, 0, 0);
-            d2[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 1) * 64 + lane) * 8), h1b,
-                                                             d2[rb], 0, 0, 0);
-        }
         const float4 *b2 = reinterpret_cast<const float4 *>(m.b2 + 4 * g);
-        const bf16x8 h2a = pack_gelu(d2[0], b2[0], d2[1], b2[4]);
-        const bf16x8 h2b = pack_gelu(d2[2], b2[8], d2[3], b2[12]);
-        __builtin_amdgcn_sched_barrier(0);
+        bf16x8 h1[2];
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w1f + ((2 * hf) * 64 + lane) * 8), xb,
+                                                                     z4, 0, 0, 0);
+            const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                load_bf16x8(m.w1f + ((2 * hf + 1) * 64 + lane) * 8), xb, z4, 0, 0, 0);
+            h1[hf] = pack_gelu(p0, b1[8 * hf], p1, b1[8 * hf + 4]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        bf16x8 h2[2];
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            f32x4 p[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int rb = 2 * hf + q;
+                p[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 0) * 64 + lane) * 8),
+                                                               h1[0], z4, 0, 0, 0);
+                p[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 1) * 64 + lane) * 8),
+                                                               h1[1], p[q], 0, 0, 0);
+            }
+            h2[hf] = pack_gelu(p[0], b2[8 * hf], p[1], b2[8 * hf + 4]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        const bf16x8 h2a = h2[0], h2b = h2[1];
         f32x4 d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w3f + lane * 8), h2a, z4, 0, 0, 0);
         d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w3f + (64 + lane) * 8), h2b, d3, 0, 0, 0);
         // outputs 0..2 of the column block's bodies sit in lanes 0..15 (g = 0)
@@ -250,14 +257,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
-    // the body's factor (alpha^2 |a_old|^2)^(-1/5) of the relative test's
-    // threshold; 0 switches it off (no previous acceleration yet, as in the
-    // force stage's kick, or alpha = 0)
-    float a5 = 0.f;
+    // alpha^2 |a_old|^2 of the relative test; +inf switches it off (no
+    // previous acceleration yet, as in the force stage's kick, or alpha = 0)
+    float aa = __builtin_inff();
     if (!kGeo && valid && step > 0 && d.alpha2_coarse > 0.f) {
         const float4 a0 = d.acc4[d.sorted[pos]];
         const float a2 = fmaf(a0.x, a0.x, fmaf(a0.y, a0.y, a0.z * a0.z));
-        if (a2 > 0.f) a5 = (float)pow((double)(d.alpha2_coarse * a2), -0.2);
+        if (a2 > 0.f) aa = d.alpha2_coarse * a2;
     }
 
     if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
@@ -308,10 +314,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         // (children are compacted to the front: past the first negative entry
         // every one is negative). A cell: the opening test of this traversal.
         // The coarse one takes the extent test first (threshold b^2 /
-        // theta_c^2 + eps^2 from summarize) and the relative test (threshold
-        // the product of the cell's and the body's factors) only when every
-        // lane passed it: both votes are wave-uniform, so the short circuit is
-        // a scalar branch.
+        // theta_c^2 + eps^2 from summarize) and works out the relative test,
+        // with the fifth power of d2, only when every lane passed it: both
+        // votes are wave-uniform, so the short circuit is a scalar branch.
         // Lane masks come straight from the compares (3 = ordered >=, 5 =
         // ordered <=). An opened cell's stack entry goes into lane `npush` of
         // `push` (a lane write, no branch on the lane); all of them are stored
@@ -332,7 +337,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             const uint64_t exec = __builtin_amdgcn_read_exec();
             bool take = __builtin_amdgcn_fcmpf(d2, kGeo ? thr_geo : mc[k].x, 3) == exec;
-            if (!kGeo && take) take = __builtin_amdgcn_fcmpf(mc[k].y * a5, d2, 5) == exec;
+            if (!kGeo && take) {
+                float p = d2 * d2;
+                p = p * p;
+                p = p * d2;
+                take = __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5) == exec;
+            }
             if (take) {
                 cells |= 1u << k;
             } else {
