@@ -10,6 +10,16 @@ namespace nb {
 constexpr float kFar = __builtin_huge_valf();
 constexpr int kStack = 512; // >= 7 * 64 + 8 entries: the deepest possible push pattern
 
+// v_writelane_b32: `old` with lane `lane` (wave-uniform) set to the
+// wave-uniform value v; one instruction (LLVM's amdgcn.writelane intrinsic,
+// which this compiler exposes under no builtin name)
+__device__ int amdgcn_writelane_i32(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+__device__ __forceinline__ unsigned lane_write(unsigned old, unsigned v, int lane)
+{
+    return (unsigned)amdgcn_writelane_i32((int)v, lane, (int)old);
+}
+
 // The opening vote of a wave: true when every active lane's predicate holds.
 // The compare result goes straight into the lane mask (no bool round trip).
 __device__ __forceinline__ bool wave_all(bool pred)

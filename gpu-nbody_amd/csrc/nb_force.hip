@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
         // Every branch in this loop is wave-uniform and none leaves it early
         // (children are compacted to the front: past the first negative entry
         // every one is negative). An opened cell's stack entry goes into lane
-        // `npush` of `push` by a select, and the opened children are stored
+        // `npush` of `push` by a lane write, and the opened children are stored
         // after the loop in one go: no lane-divergent store inside the loop,
         // which otherwise makes the compiler copy the running sums at every
         // child.
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
                 pz = fmaf(dz, f, pz);
                 ++cnt;
             } else {
-                push = lane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
+                push = lane_write(push, ((unsigned)(v - n) << 6) | (unsigned)(dep + 1), npush);
                 ++npush;
                 badv |= v - n >= cap;
             }

@@ -115,7 +115,7 @@ struct Dev {
     double *cq64;
     float4 *cqa;
     float4 *cqb;
-    float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^2) of the child's extent b
+    float2 *cmac;       // per child slot: (b^2 / theta_c^2 + eps^2, (m b^3)^(2/5)) of the child's extent b
     int *lvl_list;
     unsigned char *clvl; // cap: each cell's level, or kNotLive (level histogram -> scatter)
     int *lvl_blockhist; // kLevelBlocks * kMaxDepth

@@ -6,9 +6,12 @@
 // largest distance of its bodies from its centre of mass, bounded in
 // summarize) and mass m at s^2 = |d|^2 + eps^2 when
 //     s^2 >= b^2 / theta_c^2 + eps^2           (extent test: |d| >= b / theta_c)
-//     (m b^3)^2 <= alpha^2 |a_old|^2 s^10      (m b^3 / s^5, the size of the
-//                                               first term left out, within
-//                                               alpha |a_old|)
+//     s^2 >= (m b^3)^(2/5) (alpha^2 |a_old|^2)^(-1/5)
+//                                              (relative test: m b^3 / s^5, the
+//                                               size of the first term left
+//                                               out, within alpha |a_old|; its
+//                                               two factors are rounded to FP32
+//                                               once, per cell and per body)
 // with a_old the body's acceleration of the previous step (the relative test
 // is off for a lane while there is none). Every accepted cell contributes its
 // monopole and its quadrupole under the softened kernel: with d = centre of
@@ -257,13 +260,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const float4 me = valid ? d.pm4[d.sorted[pos]] : make_float4(kFar, kFar, kFar, 0.f);
     const float eps2 = d.eps2;
     const int cap = d.cap;
-    // alpha^2 |a_old|^2 of the relative test; +inf switches it off (no
-    // previous acceleration yet, as in the force stage's kick, or alpha = 0)
-    float aa = __builtin_inff();
+    // the body's factor (alpha^2 |a_old|^2)^(-1/5) of the relative test's
+    // threshold; 0 switches it off (no previous acceleration yet, as in the
+    // force stage's kick, or alpha = 0)
+    float a5 = 0.f;
     if (!kGeo && valid && step > 0 && d.alpha2_coarse > 0.f) {
         const float4 a0 = d.acc4[d.sorted[pos]];
         const float a2 = fmaf(a0.x, a0.x, fmaf(a0.y, a0.y, a0.z * a0.z));
-        if (a2 > 0.f) aa = d.alpha2_coarse * a2;
+        if (a2 > 0.f) a5 = (float)pow((double)(d.alpha2_coarse * a2), -0.2);
     }
 
     if (lane == 0) stk[wv][0] = 0; // root cell 0 at depth 0
@@ -314,13 +318,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         // (children are compacted to the front: past the first negative entry
         // every one is negative). A cell: the opening test of this traversal.
         // The coarse one takes the extent test first (threshold b^2 /
-        // theta_c^2 + eps^2 from summarize) and works out the relative test,
-        // with the fifth power of d2, only when every lane passed it: both
-        // votes are wave-uniform, so the short circuit is a scalar branch.
+        // theta_c^2 + eps^2 from summarize) and the relative test (threshold
+        // the product of the cell's and the body's factors) only when every
+        // lane passed it: both votes are wave-uniform, so the short circuit is
+        // a scalar branch.
         // Lane masks come straight from the compares (3 = ordered >=, 5 =
         // ordered <=). An opened cell's stack entry goes into lane `npush` of
-        // `push` (a lane write, no branch on the lane); all of them are stored
-        // after the loop.
+        // `push` (v_writelane: no compare, no branch on the lane); all of them
+        // are stored after the loop.
         unsigned bodies = 0, cells = 0;
         int npush = 0;
         bool badv = false;
@@ -337,16 +342,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             const uint64_t exec = __builtin_amdgcn_read_exec();
             bool take = __builtin_amdgcn_fcmpf(d2, kGeo ? thr_geo : mc[k].x, 3) == exec;
-            if (!kGeo && take) {
-                float p = d2 * d2;
-                p = p * p;
-                p = p * d2;
-                take = __builtin_amdgcn_fcmpf(mc[k].y, aa * p, 5) == exec;
-            }
+            if (!kGeo && take) take = __builtin_amdgcn_fcmpf(mc[k].y * a5, d2, 5) == exec;
             if (take) {
                 cells |= 1u << k;
             } else {
-                push = lane == npush ? ((unsigned)(v - n) << 6) | (unsigned)(dep + 1) : push;
+                push = lane_write(push, ((unsigned)(v - n) << 6) | (unsigned)(dep + 1), npush);
                 ++npush;
                 badv |= v - n >= cap;
             }

@@ -889,15 +889,19 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
  * when s^2 >= dq[depth] (the opening table of `theta`, as in the force stage).
  * With mac = ORC_MAC_EXTENT a lane accepts when both
  *     s^2 >= b^2 / theta^2 + eps^2        (the extent test: |d| >= b / theta)
- *     (m b^3)^2 <= alpha^2 |a_old|^2 s^10  (m b^3 / s^5, the size of the first
+ *     s^2 >= (m b^3)^(2/5) (alpha^2 |a_old|^2)^(-1/5)
+ *                                          (m b^3 / s^5, the size of the first
  *                                           term left out, within alpha |a_old|)
  * hold, with a_old the body's acceleration of the previous step (the test is
  * skipped for a lane while step <= 0, when the force stage has no previous
  * acceleration either (calculateforce.cl:174-179), when |a_old| = 0, or when
  * alpha <= 0). Opening decisions are made in
  * FP32 with the engine's operation order: s^2 = fmaf chain, thr = fmaf(b^2,
- * 1/theta^2, eps^2), s^10 = ((s^2 s^2)(s^2 s^2)) s^2, with b^2 and (m b^3)^2
- * rounded from FP64 per child; the force terms are summed in FP64. For each
+ * 1/theta^2, eps^2), the relative threshold the FP32 product of the cell's
+ * factor (m b^3)^(2/5) (FP64 pow of m b^3, rounded) and the body's factor
+ * (alpha^2 |a_old|^2)^(-1/5) (FP64 pow of the FP32 alpha^2 |a_old|^2,
+ * rounded; 0 while the test is off), with b^2 rounded from FP64 per child;
+ * the force terms are summed in FP64. For each
  * body of the listed groups it writes the acceleration (acc, 3 per body) and,
  * when feat is not NULL, the 24 features of the engine's coarse kernel and the
  * frame vector e2 (3 per body). Output row of lane l of the i-th listed group:
@@ -927,13 +931,14 @@ typedef struct coarse_cfg {
 } coarse_cfg;
 
 /* per child: the threshold fmaf(b^2, 1/theta^2, eps^2) of b^2 rounded to FP32
- * from the FP64 extent, and (m b^3)^2 rounded from FP64 */
-static void extent_terms(const orc_sim *s, int c, float it2, float *thr, float *mb3sq)
+ * from the FP64 extent, and the relative test's cell factor (m b^3)^(2/5)
+ * rounded from FP64 */
+static void extent_terms(const orc_sim *s, int c, float it2, float *thr, float *mb3f)
 {
     const double b = s->cq[8 * (size_t)c + 7], mc = s->com[4 * (size_t)c + 3];
     const double mb3 = mc * ((b * b) * b);
     *thr = fmaf((float)(b * b), it2, s->eps2);
-    *mb3sq = (float)(mb3 * mb3);
+    *mb3f = (float)pow(mb3, 0.4);
 }
 
 static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slot0, float *feat, double *acc,
@@ -946,7 +951,7 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
     const int first = g * W;
     const int cnt = (first + W <= n) ? W : n - first;
     if (cnt <= 0) return;
-    float px[256], py[256], pz[256], aa[256];
+    float px[256], py[256], pz[256], a5[256];
     /* per lane: total, bodies, quadrupole part */
     double A[256][3], Bd[256][3], Qp[256][3];
     double smc = 0.0, smb = 0.0;
@@ -957,7 +962,7 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
         px[l] = X[b]; py[l] = Y[b]; pz[l] = Z[b];
         for (int j = 0; j < 3; ++j) A[l][j] = Bd[l][j] = Qp[l][j] = 0.0;
         const float a2 = fmaf(s->ax[b], s->ax[b], fmaf(s->ay[b], s->ay[b], s->az[b] * s->az[b]));
-        aa[l] = (cfg->alpha2 > 0.0f && s->step > 0 && a2 > 0.0f) ? cfg->alpha2 * a2 : INFINITY;
+        a5[l] = (cfg->alpha2 > 0.0f && s->step > 0 && a2 > 0.0f) ? (float)pow((double)(cfg->alpha2 * a2), -0.2) : 0.0f;
     }
     long long ni = 0;
     int depth = 0;
@@ -976,9 +981,9 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
             int accept = 1;
             float thr = 0.0f;
             if (c >= n) {
-                float mb3sq = 0.0f;
+                float mb3f = 0.0f;
                 if (cfg->mac == ORC_MAC_EXTENT) {
-                    extent_terms(s, c, cfg->it2, &thr, &mb3sq);
+                    extent_terms(s, c, cfg->it2, &thr, &mb3f);
                 } else {
                     thr = cfg->dq[depth];
                 }
@@ -986,12 +991,7 @@ static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slo
                     const float dx = cxn - px[l], dy = cyn - py[l], dz = czn - pz[l];
                     const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
                     if (!(d2 >= thr)) accept = 0;
-                    if (accept && cfg->mac == ORC_MAC_EXTENT) {
-                        float p = d2 * d2;
-                        p = p * p;
-                        p = p * d2;
-                        if (!(mb3sq <= aa[l] * p)) accept = 0;
-                    }
+                    if (accept && cfg->mac == ORC_MAC_EXTENT && !(mb3f * a5[l] <= d2)) accept = 0;
                 }
             }
             if (!accept) {
