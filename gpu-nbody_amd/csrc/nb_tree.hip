@@ -388,8 +388,8 @@ __device__ __forceinline__ void summarize_cell(const Dev &d, int first, int i, i
         // of mass (a body's extent is 0); each child's own b enters the
         // coarse traversal's tests as the threshold b^2 / theta_c^2 + eps^2
         // (an FP32 fmaf of b^2 rounded from FP64) and (m b^3)^(2/5), the
-        // cell's factor of the relative test's threshold on s^2 (FP64 pow,
-        // rounded).
+        // cell's factor of the relative test's threshold on s^2 (FP64
+        // exp2(0.4 log2(m b^3)), rounded).
         // Explicit fma and no add after a product: the opening decisions that
         // follow from these values are bit-exact against the oracle.
         const double dx = x - com.x, dy = y - com.y, dz = z - com.z;
@@ -417,7 +417,7 @@ __device__ __forceinline__ void summarize_cell(const Dev &d, int first, int i, i
                 d.cqb[(size_t)c * 8 + at] = make_float4((float)q[3], (float)q[4], (float)q[5], 0.f);
                 const double mb3 = m * ((bc * bc) * bc);
                 d.cmac[(size_t)c * 8 + at] =
-                    make_float2(fmaf((float)(bc * bc), d.it2_coarse, d.eps2), (float)pow(mb3, 0.4));
+                    make_float2(fmaf((float)(bc * bc), d.it2_coarse, d.eps2), (float)exp2(0.4 * log2(mb3)));
             }
             if (k == 0) {
                 double4 *cq = reinterpret_cast<double4 *>(d.cq64 + (size_t)c * 8);

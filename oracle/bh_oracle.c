@@ -898,7 +898,7 @@ double orc_mlp_loss_grad(const double *P, const float *x, const float *t, int n,
  * alpha <= 0). Opening decisions are made in
  * FP32 with the engine's operation order: s^2 = fmaf chain, thr = fmaf(b^2,
  * 1/theta^2, eps^2), the relative threshold the FP32 product of the cell's
- * factor (m b^3)^(2/5) (FP64 pow of m b^3, rounded) and the body's factor
+ * factor (m b^3)^(2/5) (FP64 exp2(0.4 log2(m b^3)), rounded) and the body's factor
  * (alpha^2 |a_old|^2)^(-1/5) (FP64 pow of the FP32 alpha^2 |a_old|^2,
  * rounded; 0 while the test is off), with b^2 rounded from FP64 per child;
  * the force terms are summed in FP64. For each
@@ -938,7 +938,7 @@ static void extent_terms(const orc_sim *s, int c, float it2, float *thr, float *
     const double b = s->cq[8 * (size_t)c + 7], mc = s->com[4 * (size_t)c + 3];
     const double mb3 = mc * ((b * b) * b);
     *thr = fmaf((float)(b * b), it2, s->eps2);
-    *mb3f = (float)pow(mb3, 0.4);
+    *mb3f = (float)exp2(0.4 * log2(mb3));
 }
 
 static void coarse_group(const orc_sim *s, const coarse_cfg *cfg, int g, int slot0, float *feat, double *acc,
