@@ -252,9 +252,9 @@ hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
 // when PHI (else w = 0). Block (t, s) sums the bodies [s * chunk, (s + 1) *
 // chunk) for 64 sampled targets: 4 threads per target stride through the
 // chunk, whose bodies are staged through LDS (converted to FP64) once for all
-// 64 targets. 1 / sqrt(r^2) starts from the FP32 hardware estimate (relative
-// error < 2^-22 on the FP32-rounded argument) and takes one FP64 Newton step,
-// y (3/2 - r^2 y^2 / 2), which leaves a relative error below 1e-13. Partial
+// 64 targets. 1 / sqrt(r^2) starts from the FP64 hardware estimate
+// (v_rsq_f64) and takes one FP64 Newton step, y (3/2 - r^2 y^2 / 2), which
+// squares its relative error (the parity test holds it to 1e-12). Partial
 // sums per (target, chunk) are combined by a second kernel in a fixed order,
 // so the result is deterministic.
 // ---------------------------------------------------------------------------
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
             const double4 s = tile[k];
             const double dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
             const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
-            double ri = (double)__builtin_amdgcn_rsqf((float)r2);
+            double ri = __builtin_amdgcn_rsq(r2);
             ri = ri * fma(-0.5 * r2, ri * ri, 1.5);
             const double f = s.w * ri * ri * ri;
             ax = fma(dx, f, ax);

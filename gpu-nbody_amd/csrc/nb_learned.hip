@@ -137,15 +137,17 @@ static_assert(kStackC >= kStack && kStackC * 4 >= 64 * kFeat * 2, "coarse stack 
 //   B: column i16 of the 16-column block, k = 8 g .. 8 g + 7
 //   D: column i16, rows 4 g .. 4 g + 3
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bf16x8 pack_gelu(f32x4 lo, float4 blo, f32x4 hi, float4 bhi)
+// GELU of two accumulator blocks (their biases came in as the accumulators'
+// initial values), packed to BF16
+__device__ __forceinline__ bf16x8 pack_gelu(f32x4 lo, f32x4 hi)
 {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 h = {pack_bf16(gelu_infer(lo[0] + blo.x), gelu_infer(lo[1] + blo.y)),
-                     pack_bf16(gelu_infer(lo[2] + blo.z), gelu_infer(lo[3] + blo.w)),
-                     pack_bf16(gelu_infer(hi[0] + bhi.x), gelu_infer(hi[1] + bhi.y)),
-                     pack_bf16(gelu_infer(hi[2] + bhi.z), gelu_infer(hi[3] + bhi.w))};
+    const u32x4 h = {pack_bf16(gelu_infer(lo[0]), gelu_infer(lo[1])), pack_bf16(gelu_infer(lo[2]), gelu_infer(lo[3])),
+                     pack_bf16(gelu_infer(hi[0]), gelu_infer(hi[1])), pack_bf16(gelu_infer(hi[2]), gelu_infer(hi[3]))};
     return __builtin_bit_cast(bf16x8, h);
 }
+
+__device__ __forceinline__ f32x4 as_f32x4(float4 v) { return f32x4{v.x, v.y, v.z, v.w}; }
 
 // Runs the wave's 64 rows of normalized BF16 features (fs: LDS, row r at
 // fs + 24 r) through the network; lane r receives the residual of row r.
@@ -170,10 +172,10 @@ __device__ __forceinline__ void mlp_wave(const uint16_t *fs, const Mlp &m, int l
 #pragma unroll
         for (int hf = 0; hf < 2; ++hf) {
             const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w1f + ((2 * hf) * 64 + lane) * 8), xb,
-                                                                     z4, 0, 0, 0);
+                                                                     as_f32x4(b1[8 * hf]), 0, 0, 0);
             const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                load_bf16x8(m.w1f + ((2 * hf + 1) * 64 + lane) * 8), xb, z4, 0, 0, 0);
-            h1[hf] = pack_gelu(p0, b1[8 * hf], p1, b1[8 * hf + 4]);
+                load_bf16x8(m.w1f + ((2 * hf + 1) * 64 + lane) * 8), xb, as_f32x4(b1[8 * hf + 4]), 0, 0, 0);
+            h1[hf] = pack_gelu(p0, p1);
             __builtin_amdgcn_sched_barrier(0);
         }
         bf16x8 h2[2];
@@ -184,11 +186,11 @@ __device__ __forceinline__ void mlp_wave(const uint16_t *fs, const Mlp &m, int l
             for (int q = 0; q < 2; ++q) {
                 const int rb = 2 * hf + q;
                 p[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 0) * 64 + lane) * 8),
-                                                               h1[0], z4, 0, 0, 0);
+                                                               h1[0], as_f32x4(b2[4 * rb]), 0, 0, 0);
                 p[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(load_bf16x8(m.w2f + ((rb * 2 + 1) * 64 + lane) * 8),
                                                                h1[1], p[q], 0, 0, 0);
             }
-            h2[hf] = pack_gelu(p[0], b2[8 * hf], p[1], b2[8 * hf + 4]);
+            h2[hf] = pack_gelu(p[0], p[1]);
             __builtin_amdgcn_sched_barrier(0);
         }
         const bf16x8 h2a = h2[0], h2b = h2[1];
@@ -412,15 +414,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float Qz = fmaf(qb.y, dx, fmaf(qb.z, dy, qa.z * dz));
             const float dQd = fmaf(Qx, dx, fmaf(Qy, dy, Qz * dz));
             const float r5 = r3 * r2i;
-            const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
             if (kFeatOn) {
+                // the quadrupole part as r5 (u d - Q d), u = 5/2 (d.Q.d - T eps^2) / s^2
+                const float u = (dQd - qa.w) * (2.5f * r2i);
                 pmx = fmaf(dx, f, pmx);
                 pmy = fmaf(dy, f, pmy);
                 pmz = fmaf(dz, f, pmz);
-                pqx = fmaf(c2, dx, fmaf(-r5, Qx, pqx));
-                pqy = fmaf(c2, dy, fmaf(-r5, Qy, pqy));
-                pqz = fmaf(c2, dz, fmaf(-r5, Qz, pqz));
+                pqx = fmaf(r5, fmaf(u, dx, -Qx), pqx);
+                pqy = fmaf(r5, fmaf(u, dy, -Qy), pqy);
+                pqz = fmaf(r5, fmaf(u, dz, -Qz), pqz);
             } else {
+                const float c2 = (dQd - qa.w) * ((2.5f * r5) * r2i);
                 // monopole and quadrupole radial terms share one factor
                 const float h = f + c2;
                 pmx = fmaf(dx, h, fmaf(-r5, Qx, pmx));
