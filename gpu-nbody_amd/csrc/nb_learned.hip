@@ -232,6 +232,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     // a wave's traversal stack; after the traversal the learned step reuses it
     // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
     __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
+    // a popped cell's children's quadrupole records, (qa, qb) per child, for
+    // the accepted cells' uniform LDS reads
+    __shared__ float4 qst[4][16];
     __shared__ float dq_s[kMaxDepth];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -391,6 +394,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             pbz = fmaf(dz, f, pbz);
             if (kFeatOn) smb += qk.w;
         }
+        // the children's quadrupole records into this wave's LDS rows (the
+        // vector loads have had the decisions and the bodies to arrive): an
+        // accepted cell reads its two with uniform LDS reads, no VALU
+        if constexpr (kLaneQ) {
+            if (cells) {
+                if (lane < 8) {
+                    qst[wv][2 * lane] = qav;
+                    qst[wv][2 * lane + 1] = qbv;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (!((cells >> k) & 1u)) continue;
@@ -399,8 +414,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
             const float d2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             float4 qa, qb; // (Qxx, Qyy, Qzz, T eps^2), (Qxy, Qxz, Qyz, 0)
             if constexpr (kLaneQ) {
-                qa = make_float4(lane_read(qav.x, k), lane_read(qav.y, k), lane_read(qav.z, k), lane_read(qav.w, k));
-                qb = make_float4(lane_read(qbv.x, k), lane_read(qbv.y, k), lane_read(qbv.z, k), 0.f);
+                qa = qst[wv][2 * k];
+                qb = qst[wv][2 * k + 1];
             } else {
                 qa = cqa[(size_t)c * 8 + k];
                 qb = cqb[(size_t)c * 8 + k];
