@@ -215,16 +215,14 @@ __device__ __forceinline__ float lane_read(float x, int k)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
 }
 
-template <int OUT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(Dev d, Learn L, const int *__restrict__ cchild,
-                                                  const float4 *__restrict__ cdata,
-                                                  const float4 *__restrict__ cqa,
-                                                  const float4 *__restrict__ cqb,
-                                                  const float2 *__restrict__ cmac,
-                                                  const float *__restrict__ dqtab,
-                                                  const int *__restrict__ groups, int ngroups,
-                                                  float *__restrict__ xraw, float4 *__restrict__ frame_out,
-                                                  float4 *__restrict__ acc_out, int fmode)
+#define NB_COARSE_PARAMS                                                                                     \
+    Dev d, Learn L, const int *__restrict__ cchild, const float4 *__restrict__ cdata,                        \
+        const float4 *__restrict__ cqa, const float4 *__restrict__ cqb, const float2 *__restrict__ cmac,     \
+        const float *__restrict__ dqtab, const int *__restrict__ groups, int ngroups, float *__restrict__ xraw, \
+        float4 *__restrict__ frame_out, float4 *__restrict__ acc_out, int fmode
+#define NB_COARSE_PASS d, L, cchild, cdata, cqa, cqb, cmac, dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
+
+template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PARAMS)
 {
     constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
     constexpr bool kGeo = OUT == kOutAccel; // the fine traversal: the force stage's opening table
@@ -662,6 +660,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         __hip_atomic_fetch_add(&ctl->cell_interactions, (unsigned long long)ncell * nvalid, __ATOMIC_RELAXED,
                                AGENT);
     }
+}
+
+// The kernels. Six waves per SIMD at least for the network's epilogue (80
+// registers); the coarse force without it asks for seven, which its 68
+// registers allow and which changes its scalar register budget and code
+// (0.12-0.17 ms less at 4M; the learned kernel at seven spills in its
+// epilogue and is slower).
+template <int OUT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(NB_COARSE_PARAMS)
+{
+    coarse_traverse<OUT>(NB_COARSE_PASS);
+}
+
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_coarse64<kOutFinish>(NB_COARSE_PARAMS)
+{
+    coarse_traverse<kOutFinish>(NB_COARSE_PASS);
 }
 
 // One block: a histogram of the groups' cost classes (1 + floor(log2 cost),
