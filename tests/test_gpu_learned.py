@@ -49,15 +49,28 @@ def _hand_over(sim, st, step_offset=0):
     return orc
 
 
-@pytest.mark.parametrize("case", ["plummer_16k", "clusters_32k", "ragged_5000", "clusters_32k_history"])
+def _massless_half(n, seed):
+    """A ball whose bodies with x > 0.2 have no mass: the cells holding only
+    them have mass 0, their centre at the origin (where the reference's
+    summarize gives 0 * inf = NaN) and a relative-test factor of 0."""
+    from nbody_amd import universe as U
+
+    st = U.uniform_ball(n, seed=seed)
+    m = np.where(st[0] > 0.2, np.float32(0), st[6]).astype(np.float32)
+    return tuple(st[:6]) + (m,)
+
+
+@pytest.mark.parametrize("case", ["plummer_16k", "clusters_32k", "ragged_5000", "clusters_32k_history",
+                                  "massless_8k_history"])
 def test_coarse_features_match_oracle(nb, oracle, case):
-    """The `history` case runs three coarse steps first, so the relative test
+    """The `history` cases run three coarse steps first, so the relative test
     (alpha 0.02, on the previous accelerations) takes part in every decision."""
     from nbody_amd import universe as U
 
     st = {"plummer_16k": lambda: U.plummer(16384, seed=2),
           "clusters_32k": lambda: U.gaussian_clusters(32768, seed=4),
           "clusters_32k_history": lambda: U.gaussian_clusters(32768, seed=4),
+          "massless_8k_history": lambda: _massless_half(8192, 5),
           "ragged_5000": lambda: U.uniform_ball(5000, seed=9)}[case]()
     sim = _coarse_sim(st, coarse_alpha=ALPHA)
     if case.endswith("history"):
@@ -79,6 +92,13 @@ def test_coarse_features_match_oracle(nb, oracle, case):
     # engine outputs are by body index, oracle rows by sorted position
     ga, gf, ge2 = a[srt], f[srt], e2[srt]
     err = rel_err(ga, oa)
+    if case.startswith("massless"):
+        # the massive part's pull nearly cancels near its centre of mass (|a|
+        # down to 0.05 of the median), where the FP32 sums' rounding, which
+        # scales with the cancelling parts, is held to 1e-5 of half the
+        # median acceleration instead of the body's own
+        on = np.linalg.norm(oa, axis=1)
+        err = err * on / np.maximum(on, 0.5 * np.median(on))
     assert err.max() <= 1e-5, (err.max(), int(err.argmax()))
     # features 17 and 18 are logs of integer counts: the counts are the oracle's exactly
     for k in (17, 18):
