@@ -272,7 +272,9 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
     const int chunk = (n + kDirChunks - 1) / kDirChunks;
     const int lo = min(n, (int)blockIdx.y * chunk), hi = min(n, lo + chunk);
     const int tl = threadIdx.x & (kDirTargets - 1); // target of this thread
-    const int sub = threadIdx.x / kDirTargets;      // 0..3: which quarter of each tile
+    // 0..3: which quarter of each tile (the wave's index: uniform, so the
+    // source loop below is counted in scalar registers)
+    const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kDirTargets);
     const int tgt = t0 + tl;
     double tx = 0.0, ty = 0.0, tz = 0.0;
     int self = -1;
@@ -293,7 +295,8 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
         tile[threadIdx.x] = make_double4(pj.x, pj.y, pj.z, pj.w);
         __syncthreads();
         const int cnt = min(kDirTile, hi - base);
-        for (int k = sub; k < cnt; k += 4) {
+        // one source's terms, added in source order
+        auto pair = [&](int k) {
             const double4 s = tile[k];
             const double dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
             const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
@@ -304,7 +307,15 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
             ay = fma(dy, f, ay);
             az = fma(dz, f, az);
             if (PHI) phi = fma(s.w, base + k != self ? ri : 0.0, phi);
+        };
+        // two sources per round: their loads and arithmetic overlap, only the
+        // final adds are chained (the same order as one at a time)
+        int k = sub;
+        for (; k + 4 < cnt; k += 8) {
+            pair(k);
+            pair(k + 4);
         }
+        if (k < cnt) pair(k);
     }
     red[threadIdx.x] = make_double4(ax, ay, az, phi);
     __syncthreads();

@@ -226,7 +226,7 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
 {
     constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
     constexpr bool kGeo = OUT == kOutAccel; // the fine traversal: the force stage's opening table
-    constexpr bool kLaneQ = true; // quadrupole records by vector loads and lane reads
+    constexpr bool kLaneQ = true; // quadrupole records by vector loads, read back from LDS
     // a wave's traversal stack; after the traversal the learned step reuses it
     // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
     __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
@@ -310,7 +310,7 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
         }
         // the children's quadrupole records, child k in lanes k, k + 8, ...:
         // two vector loads issued now, waited for at the first accepted cell
-        // (which takes its record by lane reads), so their latency hides
+        // (which reads them from the wave's LDS), so their latency hides
         // behind the decisions and the bodies
         float4 qav, qbv;
         if constexpr (kLaneQ) {
