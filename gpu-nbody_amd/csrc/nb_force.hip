@@ -308,14 +308,16 @@ __global__ __launch_bounds__(256) void k_direct64(Dev d, const int *__restrict__
             az = fma(dz, f, az);
             if (PHI) phi = fma(s.w, base + k != self ? ri : 0.0, phi);
         };
-        // two sources per round: their loads and arithmetic overlap, only the
+        // four sources per round: their loads and arithmetic overlap, only the
         // final adds are chained (the same order as one at a time)
         int k = sub;
-        for (; k + 4 < cnt; k += 8) {
+        for (; k + 12 < cnt; k += 16) {
             pair(k);
             pair(k + 4);
+            pair(k + 8);
+            pair(k + 12);
         }
-        if (k < cnt) pair(k);
+        for (; k < cnt; k += 4) pair(k);
     }
     red[threadIdx.x] = make_double4(ax, ay, az, phi);
     __syncthreads();
