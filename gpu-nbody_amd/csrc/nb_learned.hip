@@ -311,11 +311,13 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
         // the children's quadrupole records, child k in lanes k, k + 8, ...:
         // two vector loads issued now, waited for at the first accepted cell
         // (which reads them from the wave's LDS), so their latency hides
-        // behind the decisions and the bodies
+        // behind the decisions and the bodies; each from the popped cell's
+        // uniform base plus an unsigned lane offset (fewer 64-bit address
+        // adds per pop)
         float4 qav, qbv;
         if constexpr (kLaneQ) {
-            qav = cqa[(size_t)c * 8 + (lane & 7)];
-            qbv = cqb[(size_t)c * 8 + (lane & 7)];
+            qav = (cqa + (size_t)c * 8)[(unsigned)lane & 7u];
+            qbv = (cqb + (size_t)c * 8)[(unsigned)lane & 7u];
         }
         // First the opening decisions, wave-uniform bit masks of the children
         // (children are compacted to the front: past the first negative entry

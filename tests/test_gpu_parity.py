@@ -288,11 +288,16 @@ def test_coincident_bodies_report_error(nb):
     sim.close()
 
 
-def test_direct_sum_matches_oracle(nb, oracle):
-    """The device FP64 direct sum (the error yardstick) against the oracle's."""
-    st = _universes()["clusters_32k"]()
+@pytest.mark.parametrize("case", ["clusters_32k", "ragged_12345"])
+def test_direct_sum_matches_oracle(nb, oracle, case):
+    """The device FP64 direct sum (the error yardstick) against the oracle's.
+    12,345 bodies split into 64 source chunks of 193 (the last one 186):
+    every tile is partial, so the source loop's rounds of four and its
+    remainder both run."""
+    st = _universes()[case]()
+    n = len(st[0])
     sim = make_sim(st, **PHYS)
-    idx = np.arange(0, 32768, 97, dtype=np.int32)
+    idx = np.arange(0, n, 97, dtype=np.int32)
     g = sim.direct_accel(idx)
     o = oracle.direct_fp64(st[0], st[1], st[2], st[6], 0.05, idx)
     assert np.max(np.abs(g - o) / np.linalg.norm(o, axis=1)[:, None]) < 1e-12
