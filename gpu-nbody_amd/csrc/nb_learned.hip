@@ -222,6 +222,12 @@ __device__ __forceinline__ float lane_read(float x, int k)
         float4 *__restrict__ frame_out, float4 *__restrict__ acc_out, int fmode
 #define NB_COARSE_PASS d, L, cchild, cdata, cqa, cqb, cmac, dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
 
+// waves (vote groups) per block of the coarse traversal. The waves share
+// nothing, and a block of one wave frees its slot as soon as its group is
+// done, where a block of four holds a CU's share until its slowest group
+// ends (learned force 3.48 -> 3.42 ms at 4M against four, 3.44 with two)
+constexpr int kCoarseWaves = 1;
+
 template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PARAMS)
 {
     constexpr bool kFeatOn = OUT == kOutLearned || OUT == kOutRaw;
@@ -229,10 +235,10 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
     constexpr bool kLaneQ = true; // quadrupole records by vector loads, read back from LDS
     // a wave's traversal stack; after the traversal the learned step reuses it
     // for the wave's 64 rows of BF16 features (64 x 24 x 2 bytes)
-    __shared__ __attribute__((aligned(16))) unsigned stk[4][kStackC];
+    __shared__ __attribute__((aligned(16))) unsigned stk[kCoarseWaves][kStackC];
     // a popped cell's children's quadrupole records, (qa, qb) per child, for
     // the accepted cells' uniform LDS reads
-    __shared__ float4 qst[4][16];
+    __shared__ float4 qst[kCoarseWaves][16];
     __shared__ float dq_s[kMaxDepth];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -244,12 +250,13 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
         if (tid == 0) flag_error(ctl, kErrDepth);
         return;
     }
-    if (kGeo && tid < kMaxDepth) dq_s[tid] = dqtab[tid];
+    if (kGeo)
+        for (int i = tid; i < kMaxDepth; i += 64 * kCoarseWaves) dq_s[i] = dqtab[i];
     const int step = ctl->step;
     const float root_r = ctl->radius;
     __syncthreads();
 
-    const int gslot = blockIdx.x * 4 + wv;
+    const int gslot = blockIdx.x * kCoarseWaves + wv;
     int g = gslot;
     if (groups) {
         if (gslot >= ngroups) return;
@@ -670,13 +677,13 @@ template <int OUT> __device__ __forceinline__ void coarse_traverse(NB_COARSE_PAR
 // (0.12-0.17 ms less at 4M; the learned kernel at seven spills in its
 // epilogue and is slower).
 template <int OUT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(NB_COARSE_PARAMS)
+__global__ __launch_bounds__(64 * kCoarseWaves) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_coarse64(NB_COARSE_PARAMS)
 {
     coarse_traverse<OUT>(NB_COARSE_PASS);
 }
 
 template <>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_coarse64<kOutFinish>(NB_COARSE_PARAMS)
+__global__ __launch_bounds__(64 * kCoarseWaves) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_coarse64<kOutFinish>(NB_COARSE_PARAMS)
 {
     coarse_traverse<kOutFinish>(NB_COARSE_PASS);
 }
@@ -781,7 +788,7 @@ hipError_t launch_coarse(const Dev &d, const Learn &L, int out, const float *dqt
 {
     const int waves = groups ? ngroups : (d.n + 63) / 64;
     if (waves <= 0) return hipSuccess;
-    const dim3 grid((waves + 3) / 4), block(256);
+    const dim3 grid((waves + kCoarseWaves - 1) / kCoarseWaves), block(64 * kCoarseWaves);
 #define NB_COARSE_ARGS                                                                             \
     d, L, (const int *)d.cchild, (const float4 *)d.cdata, (const float4 *)d.cqa, (const float4 *)d.cqb, \
         (const float2 *)d.cmac, dqtab, groups, ngroups, xraw, frame_out, acc_out, fmode
