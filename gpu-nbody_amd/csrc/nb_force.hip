@@ -1,7 +1,8 @@
 // nb_force.hip -- force traversal (calculateforce.cl:25-188) for gfx950, and
 // the FP64 direct sum used to measure force errors.
 //
-// A block of 256 threads covers 256 consecutive sorted positions. A vote group
+// A block covers consecutive sorted positions (256 for votes 16 and 32, one
+// wave of 64 for vote 64). A vote group
 // is G consecutive lanes (G = 16, 32 or 64; 16 is the reference's work-group
 // vote of calculateforce.cl:145 and 64 is one wave). Every group owns one
 // stack in LDS; entries are (cell << 6 | depth). Popping a cell tests all of
@@ -24,12 +25,17 @@ namespace nb {
 
 
 // ---------------------------------------------------------------------------
-// Vote width 64: one wave, one group, uniform traversal.
+// Vote width 64: one wave, one group, uniform traversal. kForceWaves waves
+// (groups) per block: they share nothing, and a block of one wave frees its
+// slot as soon as its group is done, where a block of four holds its CU's
+// share until its slowest group ends (8.97 -> 8.67 ms at 4M clustered bodies).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ cchild,
+constexpr int kForceWaves = 1;
+
+__global__ __launch_bounds__(64 * kForceWaves) void k_force64(Dev d, const int *__restrict__ cchild,
                                                  const float4 *__restrict__ cdata, int mode)
 {
-    __shared__ unsigned stk[4][kStack];
+    __shared__ unsigned stk[kForceWaves][kStack];
     __shared__ float dq_s[kMaxDepth];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -41,11 +47,11 @@ __global__ __launch_bounds__(256) void k_force64(Dev d, const int *__restrict__ 
         if (tid == 0) flag_error(ctl, kErrDepth);
         return;
     }
-    if (tid < kMaxDepth) dq_s[tid] = ctl->dq[tid];
+    for (int i = tid; i < kMaxDepth; i += 64 * kForceWaves) dq_s[i] = ctl->dq[i];
     const int step = ctl->step;
     __syncthreads();
 
-    const int pos = blockIdx.x * 256 + tid;
+    const int pos = blockIdx.x * (64 * kForceWaves) + tid;
     const bool valid = pos < n;
     // a lane past the last body sits at infinity: its d^2 is infinite, so it
     // accepts every cell and never holds up the vote (its sums are discarded)
@@ -239,7 +245,8 @@ hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
     case 16: hipLaunchKernelGGL(k_force_g<16>, grid, block, 0, st, d, mode); break;
     case 32: hipLaunchKernelGGL(k_force_g<32>, grid, block, 0, st, d, mode); break;
     case 64:
-        hipLaunchKernelGGL(k_force64, grid, block, 0, st, d, (const int *)d.cchild, (const float4 *)d.cdata, mode);
+        hipLaunchKernelGGL(k_force64, dim3((d.n + 64 * kForceWaves - 1) / (64 * kForceWaves)), dim3(64 * kForceWaves),
+                           0, st, d, (const int *)d.cchild, (const float4 *)d.cdata, mode);
         break;
     default: return hipErrorInvalidValue;
     }
