@@ -1,8 +1,7 @@
 // nb_force.hip -- force traversal (calculateforce.cl:25-188) for gfx950, and
 // the FP64 direct sum used to measure force errors.
 //
-// A block covers consecutive sorted positions (256 for votes 16 and 32, one
-// wave of 64 for vote 64). A vote group
+// A block is one wave: 64 consecutive sorted positions. A vote group
 // is G consecutive lanes (G = 16, 32 or 64; 16 is the reference's work-group
 // vote of calculateforce.cl:145 and 64 is one wave). Every group owns one
 // stack in LDS; entries are (cell << 6 | depth). Popping a cell tests all of
@@ -141,10 +140,15 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force64(Dev d, const int *
 // ---------------------------------------------------------------------------
 // Vote width 16 or 32: several groups per wave, each with its own stack.
 // ---------------------------------------------------------------------------
+// threads per block of the vote-16/32 traversal: one wave (its groups share
+// nothing; blocks of 256 held a CU's share until their slowest group ended:
+// vote 16 27.6 -> 25.6 ms, vote 32 18.6 -> 18.0 ms at 4M clustered bodies)
+constexpr int kGThreads = 64;
+
 template <int G>
-__global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
+__global__ __launch_bounds__(kGThreads) void k_force_g(Dev d, int mode)
 {
-    constexpr int kGroups = 256 / G;
+    constexpr int kGroups = kGThreads / G;
     __shared__ unsigned stk[kGroups][kStack];
     __shared__ float dq_s[kMaxDepth];
     const int tid = threadIdx.x;
@@ -157,11 +161,11 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
         if (tid == 0) flag_error(ctl, kErrDepth);
         return;
     }
-    if (tid < kMaxDepth) dq_s[tid] = ctl->dq[tid];
+    for (int i = tid; i < kMaxDepth; i += kGThreads) dq_s[i] = ctl->dq[i];
     const int step = ctl->step;
     __syncthreads();
 
-    const int pos = blockIdx.x * 256 + tid;
+    const int pos = blockIdx.x * kGThreads + tid;
     const bool valid = pos < n;
     // a lane past the last body sits at infinity: its d^2 is infinite, so it
     // accepts every cell and never holds up the vote (its sums are discarded)
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(256) void k_force_g(Dev d, int mode)
 
 hipError_t launch_force(const Dev &d, int vote, int mode, hipStream_t st)
 {
-    const dim3 grid((d.n + 255) / 256), block(256);
+    const dim3 grid((d.n + kGThreads - 1) / kGThreads), block(kGThreads);
     switch (vote) {
     case 16: hipLaunchKernelGGL(k_force_g<16>, grid, block, 0, st, d, mode); break;
     case 32: hipLaunchKernelGGL(k_force_g<32>, grid, block, 0, st, d, mode); break;
